@@ -1,0 +1,942 @@
+// C ABI of libmonte.so (declared in include/monte.h): argument checking,
+// workspaces, constant caching, host-pointer staging, MC job construction,
+// autotuning and hipGraph capture.  All device work is in fdk.hip and mc.hip.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+
+using namespace monte;
+
+namespace monte {
+
+int fail(monte_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+void* workspace(monte_ctx* c, int slot, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (c->ws_size[slot] >= bytes) return c->ws[slot];
+  if (c->ws[slot]) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->ws[slot]);
+    c->ws[slot] = nullptr;
+    c->ws_size[slot] = 0;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  c->ws[slot] = p;
+  c->ws_size[slot] = bytes;
+  return p;
+}
+
+}  // namespace monte
+
+namespace {
+
+constexpr int kSlotFwd = 0, kSlotAdj = 1, kSlotMc = 2, kSlotConst = 3, kSlotStageA = 4, kSlotStageB = 5,
+              kSlotTuneA = 6, kSlotTuneB = 7;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int check_geom(monte_ctx* c, const monte_geom* g, bool need_volume) {
+  if (!g) return fail(c, MONTE_ERR_INVALID, "geometry is null");
+  if (g->preset != MONTE_PRESET_STANDARD && g->preset != MONTE_PRESET_BP3D20)
+    return fail(c, MONTE_ERR_INVALID, "unknown preset");
+  if (g->n_views < 1 || g->nu < 1 || g->nv < 1) return fail(c, MONTE_ERR_INVALID, "detector/view counts must be >= 1");
+  if (g->nu > 1024) return fail(c, MONTE_ERR_UNSUPPORTED, "detector rows longer than 1024 pixels are not supported");
+  if (!(g->sod > 0) || !(g->sdd > g->sod)) return fail(c, MONTE_ERR_INVALID, "need 0 < sod < sdd");
+  if (!(g->du > 0) || !(g->dv > 0)) return fail(c, MONTE_ERR_INVALID, "detector pitch must be > 0");
+  if (g->preset == MONTE_PRESET_BP3D20 && g->step_deg < 1) return fail(c, MONTE_ERR_INVALID, "step_deg must be >= 1");
+  if (g->preset == MONTE_PRESET_BP3D20 && g->nu != g->nv)
+    return fail(c, MONTE_ERR_INVALID, "the bp3d20 preset indexes a square detector (nu == nv)");
+  if (g->preset == MONTE_PRESET_STANDARD && g->filter != MONTE_FILTER_RAMLAK && g->filter != MONTE_FILTER_SHEPP_LOGAN)
+    return fail(c, MONTE_ERR_INVALID, "unknown filter");
+  if (need_volume) {
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(c, MONTE_ERR_INVALID, "volume dims must be >= 1");
+    if (!(g->vx > 0) || !(g->vy > 0) || !(g->vz > 0)) return fail(c, MONTE_ERR_INVALID, "voxel size must be > 0");
+    if (g->preset == MONTE_PRESET_BP3D20 && (g->roi_x0 < 0 || g->roi_x1 > g->nx || g->roi_x0 >= g->roi_x1))
+      return fail(c, MONTE_ERR_INVALID, "roi_x0/roi_x1 outside the volume");
+    const double rx = 0.5 * g->nx * g->vx + std::fabs(g->ox), ry = 0.5 * g->ny * g->vy + std::fabs(g->oy);
+    if (std::sqrt(rx * rx + ry * ry) >= g->sod) return fail(c, MONTE_ERR_INVALID, "volume reaches the source orbit");
+  }
+  return MONTE_OK;
+}
+
+std::vector<double> geom_key(const monte_geom& g) {
+  return {(double)g.preset, (double)g.n_views, (double)g.nu, (double)g.nv, (double)g.nx, (double)g.ny,
+          (double)g.nz, g.sod, g.sdd, g.du, g.dv, g.u_off, g.v_off, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz,
+          g.angle0, g.dangle, g.odd, g.scale, (double)g.step_deg, (double)g.roi_x0, (double)g.roi_x1, g.mask_r,
+          (double)g.filter};
+}
+
+// Upload filter taps and per-view coefficients once per geometry.
+int ensure_constants(monte_ctx* c, const monte_geom& g, const float** taps, const BpView** views) {
+  std::vector<double> key = geom_key(g);
+  if (key != c->const_key || !c->ws[kSlotConst]) {
+    std::vector<float> h;
+    compute_filter_taps(g, h);
+    std::vector<BpView> v;
+    compute_bp_views(g, v);
+    const size_t toff = 0, voff = align_up(h.size() * sizeof(float), 256);
+    const size_t bytes = voff + v.size() * sizeof(BpView);
+    char* base = (char*)workspace(c, kSlotConst, bytes);
+    if (!base) return fail(c, MONTE_ERR_NOMEM, "constants allocation failed");
+    MONTE_HIP(c, hipMemcpyAsync(base + toff, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    MONTE_HIP(c, hipMemcpyAsync(base + voff, v.data(), v.size() * sizeof(BpView), hipMemcpyHostToDevice, c->stream));
+    MONTE_HIP(c, hipStreamSynchronize(c->stream));
+    c->const_key = key;
+    c->taps_off = toff;
+    c->views_off = voff;
+  }
+  *taps = (const float*)((char*)c->ws[kSlotConst] + c->taps_off);
+  *views = (const BpView*)((char*)c->ws[kSlotConst] + c->views_off);
+  return MONTE_OK;
+}
+
+size_t proj_elems(const monte_geom& g) { return (size_t)g.n_views * g.nu * g.nv; }
+size_t vol_elems(const monte_geom& g) { return (size_t)g.nx * g.ny * g.nz; }
+size_t padded_elems(const monte_geom& g) {
+  return (size_t)g.n_views * (g.nu + kPadLo + kPadHi) * (g.nv + kPadLo + kPadHi);
+}
+
+// Host-pointer mode: copy an input to a staging slot / bring an output back.
+struct Stage {
+  monte_ctx* c;
+  int code = MONTE_OK;
+  const void* in(int slot, const void* host, size_t bytes) {
+    if (!c->host_ptrs) return host;
+    void* d = workspace(c, slot, bytes);
+    if (!d) { code = fail(c, MONTE_ERR_NOMEM, "staging allocation failed"); return nullptr; }
+    if (hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+      code = fail(c, MONTE_ERR_HIP, "staging copy failed");
+      return nullptr;
+    }
+    return d;
+  }
+  void* out(int slot, void* host, size_t bytes, bool copy_in) {
+    if (!c->host_ptrs) return host;
+    void* d = workspace(c, slot, bytes);
+    if (!d) { code = fail(c, MONTE_ERR_NOMEM, "staging allocation failed"); return nullptr; }
+    if (copy_in && hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+      code = fail(c, MONTE_ERR_HIP, "staging copy failed");
+      return nullptr;
+    }
+    return d;
+  }
+  int back(void* host, const void* dev, size_t bytes) {
+    if (!c->host_ptrs) return MONTE_OK;
+    MONTE_HIP(c, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    MONTE_HIP(c, hipStreamSynchronize(c->stream));
+    return MONTE_OK;
+  }
+};
+
+// ------------------------------------------------------------ FDK stages
+int run_filter_padded(monte_ctx* c, const monte_geom& g, const float* proj, float* pad) {
+  const float* taps;
+  const BpView* views;
+  int rc = ensure_constants(c, g, &taps, &views);
+  if (rc) return rc;
+  const FilterParams fp = compute_filter_params(g);
+  MONTE_HIP(c, launch_fdk_filter(proj, pad, taps, fp, c->stream));
+  MONTE_HIP(c, launch_fill_pads(pad, g.n_views, g.nu, g.nv, fp.pitch, fp.rows, g.preset == MONTE_PRESET_BP3D20,
+                                c->stream));
+  return MONTE_OK;
+}
+
+int run_backproject_padded(monte_ctx* c, const monte_geom& g, const float* pad, float* vol, int accumulate) {
+  const float* taps;
+  const BpView* views;
+  int rc = ensure_constants(c, g, &taps, &views);
+  if (rc) return rc;
+  const BpGrid grid = compute_bp_grid(g);
+  if (!accumulate && (grid.x0 > 0 || grid.x1 < g.nx)) {
+    MONTE_HIP(c, launch_fill(vol, 0.f, vol_elems(g), c->stream));
+    accumulate = 1;  // ROI columns added onto zeros
+  }
+  const int chunk = c->tuning.bp_view_chunk > 0 ? c->tuning.bp_view_chunk : g.n_views;
+  for (int v0 = 0; v0 < g.n_views; v0 += chunk) {
+    const int n = std::min(chunk, g.n_views - v0);
+    MONTE_HIP(c, launch_backproject(pad, views, v0, n, grid, vol, (accumulate || v0 > 0) ? 1 : 0, c->tuning,
+                                    c->stream));
+  }
+  return MONTE_OK;
+}
+
+int run_project_padded(monte_ctx* c, const monte_geom& g, const float* vol, float* pad) {
+  const float* taps;
+  const BpView* views;
+  int rc = ensure_constants(c, g, &taps, &views);
+  if (rc) return rc;
+  const BpGrid grid = compute_bp_grid(g);
+  MONTE_HIP(c, launch_fill(pad, 0.f, padded_elems(g), c->stream));
+  const int chunk = c->tuning.bp_view_chunk > 0 ? c->tuning.bp_view_chunk : g.n_views;
+  for (int v0 = 0; v0 < g.n_views; v0 += chunk) {
+    const int n = std::min(chunk, g.n_views - v0);
+    MONTE_HIP(c, launch_project(vol, views, v0, n, grid, pad, c->tuning, c->stream));
+  }
+  if (g.preset == MONTE_PRESET_BP3D20)
+    MONTE_HIP(c, launch_fold_pads(pad, g.n_views, g.nu, g.nv, grid.pitch, grid.rows, c->stream));
+  return MONTE_OK;
+}
+
+int run_filter_T_padded(monte_ctx* c, const monte_geom& g, const float* pad, float* proj) {
+  const float* taps;
+  const BpView* views;
+  int rc = ensure_constants(c, g, &taps, &views);
+  if (rc) return rc;
+  const FilterParams fp = compute_filter_params(g);
+  MONTE_HIP(c, launch_fdk_filter_T(pad, proj, taps, fp, c->stream));
+  return MONTE_OK;
+}
+
+// ------------------------------------------------------------------ MC
+struct HostJob {
+  McJob d{};
+  McTables tables{};
+  std::vector<float> cdf;       // empty = mono
+  std::vector<float2> view_cs;  // all views of the geometry
+};
+
+int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const monte_mc_cfg* cfg, HostJob& hj,
+              int batch) {
+  int rc = check_geom(c, g, false);
+  if (rc) return rc;
+  if (!ph || !cfg) return fail(c, MONTE_ERR_INVALID, "phantom/config is null");
+  if (!ph->labels) return fail(c, MONTE_ERR_INVALID, "phantom labels are null");
+  if (ph->nx < 1 || ph->ny < 1 || ph->nz < 1 || !(ph->vx > 0) || !(ph->vy > 0) || !(ph->vz > 0))
+    return fail(c, MONTE_ERR_INVALID, "bad phantom dimensions");
+  if (ph->n_labels < 1 || ph->n_labels > kMaxLabels) return fail(c, MONTE_ERR_INVALID, "n_labels out of range");
+  if (c->n_tables < 1) return fail(c, MONTE_ERR_STATE, "attenuation tables not loaded (monte_load_tables)");
+  if (cfg->photons_per_view == 0 || cfg->photons_per_view > 0xFFFFFFFFull)
+    return fail(c, MONTE_ERR_INVALID, "photons_per_view must be in [1, 2^32)");
+  if (cfg->max_interactions < 0) return fail(c, MONTE_ERR_INVALID, "max_interactions < 0");
+  if (!cfg->spectrum && !(cfg->e_kev >= 1.0 && cfg->e_kev <= kRows - 1))
+    return fail(c, MONTE_ERR_INVALID, "e_kev must be in [1, 200]");
+  const double half_x = 0.5 * ph->nx * ph->vx + std::fabs(ph->ox), half_y = 0.5 * ph->ny * ph->vy + std::fabs(ph->oy);
+  if (std::sqrt(half_x * half_x + half_y * half_y) >= g->sod)
+    return fail(c, MONTE_ERR_INVALID, "phantom reaches the source orbit");
+  if (std::sqrt(half_x * half_x + half_y * half_y) >= g->sdd - g->sod)
+    return fail(c, MONTE_ERR_INVALID, "phantom reaches the detector plane");
+  int vb = cfg->view_begin, ve = cfg->view_end;
+  if (vb == 0 && ve == 0) ve = g->n_views;
+  if (vb < 0 || ve > g->n_views || vb >= ve) return fail(c, MONTE_ERR_INVALID, "view range outside the scan");
+
+  // tables: mu = density * total; fractions from the mass coefficients
+  McTables& T = hj.tables;
+  std::memset(&T, 0, sizeof T);
+  for (int e = 0; e < kRows; ++e) {
+    float mx = 0.f;
+    for (int l = 0; l < ph->n_labels; ++l) {
+      const int tb = ph->mat_table[l];
+      if (tb < 0) continue;
+      if (tb >= c->n_tables) return fail(c, MONTE_ERR_INVALID, "mat_table index beyond the loaded tables");
+      if (!(ph->density[l] >= 0)) return fail(c, MONTE_ERR_INVALID, "negative density");
+      const double* t = c->tables.data() + (size_t)tb * 4 * kRows;
+      const double tot = t[3 * kRows + e];
+      float4 m;
+      m.x = (float)(ph->density[l] * tot);
+      m.y = tot > 0 ? (float)(t[2 * kRows + e] / tot) : 1.f;
+      m.z = tot > 0 ? (float)((t[2 * kRows + e] + t[0 * kRows + e]) / tot) : 1.f;
+      m.w = 0.f;
+      T.m[l][e] = m;
+      mx = std::max(mx, m.x);
+    }
+    T.mumax[e] = mx;
+  }
+  hj.cdf.clear();
+  if (cfg->spectrum) {
+    double tot = 0;
+    for (int e = 1; e < kRows; ++e) {
+      if (!(cfg->spectrum[e] >= 0)) return fail(c, MONTE_ERR_INVALID, "negative spectrum weight");
+      tot += cfg->spectrum[e];
+    }
+    if (!(tot > 0)) return fail(c, MONTE_ERR_INVALID, "spectrum has no weight above 0 keV");
+    hj.cdf.assign(kRows, 0.f);
+    double run = 0;
+    for (int e = 1; e < kRows; ++e) {
+      run += cfg->spectrum[e];
+      hj.cdf[e] = (float)(run / tot);
+    }
+    hj.cdf[kRows - 1] = 1.0f;
+  }
+  hj.view_cs.resize(g->n_views);
+  for (int k = 0; k < g->n_views; ++k) {
+    const double b = g->preset == MONTE_PRESET_BP3D20 ? 3.14159265358979323846 * (k * g->step_deg) / 180.0
+                                                      : g->angle0 + k * g->dangle;
+    hj.view_cs[k] = make_float2((float)std::cos(b), (float)std::sin(b));
+  }
+  McJob& J = hj.d;
+  J.labels = ph->labels;
+  J.nx = ph->nx; J.ny = ph->ny; J.nz = ph->nz;
+  J.nu = g->nu; J.nv = g->nv;
+  J.view_begin = vb;
+  J.n_views_sim = ve - vb;
+  J.vx = (float)ph->vx; J.vy = (float)ph->vy; J.vz = (float)ph->vz;
+  J.bx0 = (float)(ph->ox - 0.5 * ph->nx * ph->vx); J.bx1 = (float)(ph->ox + 0.5 * ph->nx * ph->vx);
+  J.by0 = (float)(ph->oy - 0.5 * ph->ny * ph->vy); J.by1 = (float)(ph->oy + 0.5 * ph->ny * ph->vy);
+  J.bz0 = (float)(ph->oz - 0.5 * ph->nz * ph->vz); J.bz1 = (float)(ph->oz + 0.5 * ph->nz * ph->vz);
+  J.inv_vx = (float)(1.0 / ph->vx); J.inv_vy = (float)(1.0 / ph->vy); J.inv_vz = (float)(1.0 / ph->vz);
+  J.sod = (float)g->sod; J.sdd = (float)g->sdd;
+  if (g->preset == MONTE_PRESET_BP3D20) {
+    // CBCT_real325.cu:565-566: index -(int)(10 d - 162.5), i.e. u_idx = nu/2 - y'/du
+    J.un0 = (float)(0.5 * g->nu); J.us = (float)(-1.0 / g->du); J.u_step = (float)(-g->du);
+    J.vn0 = (float)(0.5 * g->nv); J.vs = (float)(-1.0 / g->dv); J.v_step = (float)(-g->dv);
+  } else {
+    J.un0 = (float)(0.5 * g->nu - g->u_off / g->du); J.us = (float)(1.0 / g->du); J.u_step = (float)g->du;
+    J.vn0 = (float)(0.5 * g->nv - g->v_off / g->dv); J.vs = (float)(1.0 / g->dv); J.v_step = (float)g->dv;
+  }
+  J.e_kev = (float)cfg->e_kev;
+  J.max_int = cfg->max_interactions;
+  J.coherent = cfg->coherent_mode == MONTE_COH_RAYLEIGH ? 1 : 0;
+  J.key0 = (uint32_t)(cfg->seed & 0xFFFFFFFFull);
+  J.key1 = (uint32_t)(cfg->seed >> 32);
+  J.job_id = cfg->job;
+  J.photons_per_view = (uint32_t)cfg->photons_per_view;
+  J.batches_per_view = (uint32_t)((cfg->photons_per_view + batch - 1) / batch);
+  J.unit_count = (uint64_t)J.batches_per_view * (uint64_t)J.n_views_sim;
+  return MONTE_OK;
+}
+
+// Lay out jobs, tables, CDFs and view tables in the MC slot and launch.
+int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blocks_per_cu, float* ms_out,
+           std::vector<McCounters>* counters_out) {
+  const int n = (int)hj.size();
+  size_t off = 0;
+  const size_t jobs_off = off; off = align_up(off + sizeof(McJob) * n, 256);
+  const size_t tab_off = off; off = align_up(off + sizeof(McTables) * n, 256);
+  std::vector<size_t> cdf_off(n), cs_off(n);
+  for (int i = 0; i < n; ++i) {
+    cdf_off[i] = off; off = align_up(off + sizeof(float) * hj[i].cdf.size(), 256);
+    cs_off[i] = off; off = align_up(off + sizeof(float2) * hj[i].view_cs.size(), 256);
+  }
+  const size_t q_off = off; off = align_up(off + 256, 256);
+  const size_t cnt_off = off; off = align_up(off + sizeof(McCounters) * n, 256);
+  char* base = (char*)workspace(c, kSlotMc, off);
+  if (!base) return fail(c, MONTE_ERR_NOMEM, "MC descriptor allocation failed");
+
+  uint64_t units = 0;
+  std::vector<McJob> jobs(n);
+  for (int i = 0; i < n; ++i) {
+    McJob J = hj[i].d;
+    J.tables = (const McTables*)(base + tab_off) + i;
+    J.spectrum_cdf = hj[i].cdf.empty() ? nullptr : (const float*)(base + cdf_off[i]);
+    J.view_cs = (const float2*)(base + cs_off[i]);
+    J.unit_begin = units;
+    units += J.unit_count;
+    jobs[i] = J;
+  }
+  if (units > 0xFFFFFFFFull) return fail(c, MONTE_ERR_INVALID, "too many work units in one launch (raise the batch)");
+  MONTE_HIP(c, hipMemcpyAsync(base + jobs_off, jobs.data(), sizeof(McJob) * n, hipMemcpyHostToDevice, c->stream));
+  for (int i = 0; i < n; ++i) {
+    MONTE_HIP(c, hipMemcpyAsync(base + tab_off + sizeof(McTables) * i, &hj[i].tables, sizeof(McTables),
+                                hipMemcpyHostToDevice, c->stream));
+    if (!hj[i].cdf.empty())
+      MONTE_HIP(c, hipMemcpyAsync(base + cdf_off[i], hj[i].cdf.data(), sizeof(float) * hj[i].cdf.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+    MONTE_HIP(c, hipMemcpyAsync(base + cs_off[i], hj[i].view_cs.data(), sizeof(float2) * hj[i].view_cs.size(),
+                                hipMemcpyHostToDevice, c->stream));
+  }
+  unsigned int* queue = (unsigned int*)(base + q_off);
+  McCounters* counters = (McCounters*)(base + cnt_off);
+  MONTE_HIP(c, launch_mc_reset(queue, counters, n, c->stream));
+  hipEvent_t e0, e1;
+  MONTE_HIP(c, hipEventCreate(&e0));
+  MONTE_HIP(c, hipEventCreate(&e1));
+  MONTE_HIP(c, hipEventRecord(e0, c->stream));
+  const int grid = std::max(1, c->n_cu * blocks_per_cu);
+  hipError_t le = launch_mc_transport((const McJob*)(base + jobs_off), n, units, queue, counters, batch, block, grid,
+                                      c->stream);
+  MONTE_HIP(c, hipEventRecord(e1, c->stream));
+  std::vector<McCounters> cnt(n);
+  MONTE_HIP(c, hipMemcpyAsync(cnt.data(), counters, sizeof(McCounters) * n, hipMemcpyDeviceToHost, c->stream));
+  MONTE_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (le != hipSuccess) return fail(c, MONTE_ERR_HIP, std::string("transport launch: ") + hipGetErrorString(le));
+  if (ms_out) *ms_out = ms;
+  if (counters_out) *counters_out = cnt;
+  return MONTE_OK;
+}
+
+void fill_stats(monte_mc_stats* s, const McCounters& k, double ms) {
+  s->histories = k.histories; s->steps = k.steps; s->interactions = k.interactions;
+  s->photo = k.photo; s->coherent = k.coherent; s->compton = k.compton;
+  s->tallied_primary = k.tallied_primary; s->tallied_scatter = k.tallied_scatter;
+  s->missed = k.missed; s->capped = k.capped;
+  s->kernel_ms = ms;
+}
+
+monte_tuning default_tuning() {
+  monte_tuning t{};
+  t.mc_block = 256;
+  t.mc_batch = 16;
+  t.mc_blocks_per_cu = 4;
+  t.bp_zchunk = 8;
+  t.bp_tile_x = 16;
+  t.bp_tile_y = 16;
+  t.bp_view_chunk = 0;
+  return t;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* monte_status_str(int s) {
+  switch (s) {
+    case MONTE_OK: return "ok";
+    case MONTE_ERR_INVALID: return "invalid argument";
+    case MONTE_ERR_HIP: return "HIP runtime error";
+    case MONTE_ERR_NOMEM: return "out of device memory";
+    case MONTE_ERR_STATE: return "missing prerequisite";
+    case MONTE_ERR_IO: return "file error";
+    case MONTE_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+const char* monte_last_error(const monte_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int monte_create(monte_ctx** out, int device) {
+  if (!out) return MONTE_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return MONTE_ERR_HIP;
+  if (device < 0 || device >= n) return MONTE_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return MONTE_ERR_HIP;
+  monte_ctx* c = new monte_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MONTE_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  int cu = 0;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->n_cu = cu;
+  c->tuning = default_tuning();
+  *out = c;
+  return MONTE_OK;
+}
+
+void monte_destroy(monte_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (int i = 0; i < monte_ctx::kSlots; ++i)
+    if (c->ws[i]) (void)hipFree(c->ws[i]);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+int monte_set_stream(monte_ctx* c, void* stream) {
+  if (!c) return MONTE_ERR_INVALID;
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return MONTE_OK;
+}
+
+int monte_set_pointer_mode(monte_ctx* c, int host_pointers) {
+  if (!c) return MONTE_ERR_INVALID;
+  c->host_ptrs = host_pointers != 0;
+  return MONTE_OK;
+}
+
+int monte_synchronize(monte_ctx* c) {
+  if (!c) return MONTE_ERR_INVALID;
+  MONTE_HIP(c, hipStreamSynchronize(c->stream));
+  return MONTE_OK;
+}
+
+int monte_get_tuning(const monte_ctx* c, monte_tuning* out) {
+  if (!c || !out) return MONTE_ERR_INVALID;
+  *out = c->tuning;
+  return MONTE_OK;
+}
+
+int monte_set_tuning(monte_ctx* c, const monte_tuning* in) {
+  if (!c || !in) return MONTE_ERR_INVALID;
+  monte_tuning t = *in;
+  if (t.mc_block <= 0 || t.mc_block > 256 || t.mc_block % 64) return fail(c, MONTE_ERR_INVALID, "mc_block");
+  if (t.mc_batch <= 0 || t.mc_batch > 4096) return fail(c, MONTE_ERR_INVALID, "mc_batch");
+  if (t.mc_blocks_per_cu <= 0 || t.mc_blocks_per_cu > 64) return fail(c, MONTE_ERR_INVALID, "mc_blocks_per_cu");
+  if (t.bp_zchunk != 4 && t.bp_zchunk != 8 && t.bp_zchunk != 16) return fail(c, MONTE_ERR_INVALID, "bp_zchunk");
+  if (t.bp_tile_x <= 0 || 256 % t.bp_tile_x) return fail(c, MONTE_ERR_INVALID, "bp_tile_x");
+  t.bp_tile_y = 256 / t.bp_tile_x;
+  if (t.bp_view_chunk < 0) return fail(c, MONTE_ERR_INVALID, "bp_view_chunk");
+  c->tuning = t;
+  return MONTE_OK;
+}
+
+int monte_read_xcom_csv(const char* path, double* tab) {
+  if (!path || !tab) return MONTE_ERR_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return MONTE_ERR_IO;
+  std::memset(tab, 0, sizeof(double) * 4 * kRows);
+  char line[512];
+  int k = 0;
+  while (std::fgets(line, sizeof line, f)) {
+    char* p = line;
+    if ((unsigned char)p[0] == 0xEF && (unsigned char)p[1] == 0xBB && (unsigned char)p[2] == 0xBF) p += 3;
+    double v[4];
+    int n = 0;
+    char* q = p;
+    while (n < 4) {
+      char* e;
+      v[n] = std::strtod(q, &e);
+      if (e == q) break;
+      ++n;
+      q = e;
+      while (*q == ',' || *q == ' ' || *q == '\t') ++q;
+    }
+    if (n != 4) continue;  // blank or trailing CR-only line
+    if (k + 1 >= kRows) break;
+    ++k;
+    for (int col = 0; col < 4; ++col) tab[col * kRows + k] = v[col];
+  }
+  std::fclose(f);
+  if (k == 0) return MONTE_ERR_IO;
+  for (int col = 0; col < 4; ++col) tab[col * kRows] = tab[col * kRows + 1];
+  return k;
+}
+
+int monte_load_tables(monte_ctx* c, const double* tab, int n_tables) {
+  if (!c || !tab || n_tables < 1) return MONTE_ERR_INVALID;
+  for (size_t i = 0; i < (size_t)n_tables * 4 * kRows; ++i)
+    if (!(tab[i] >= 0)) return fail(c, MONTE_ERR_INVALID, "attenuation tables must be >= 0");
+  c->tables.assign(tab, tab + (size_t)n_tables * 4 * kRows);
+  c->n_tables = n_tables;
+  return MONTE_OK;
+}
+
+// --------------------------------------------------------------------- FDK
+int monte_fdk_filter(monte_ctx* c, const monte_geom* g, const float* proj, float* filt, int batch) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, false);
+  if (rc) return rc;
+  if (!proj || !filt || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g);
+  float* pad = (float*)workspace(c, kSlotFwd, padded_elems(*g) * sizeof(float));
+  if (!pad) return fail(c, MONTE_ERR_NOMEM, "filtered stack allocation failed");
+  Stage st{c};
+  const float* din = (const float*)st.in(kSlotStageA, proj, np * batch * sizeof(float));
+  float* dout = (float*)st.out(kSlotStageB, filt, np * batch * sizeof(float), false);
+  if (st.code) return st.code;
+  const FilterParams fp = compute_filter_params(*g);
+  for (int b = 0; b < batch; ++b) {
+    if ((rc = run_filter_padded(c, *g, din + b * np, pad))) return rc;
+    MONTE_HIP(c, launch_unpad_copy(pad, dout + b * np, g->n_views, g->nu, g->nv, fp.pitch, fp.rows, c->stream));
+  }
+  return st.back(filt, dout, np * batch * sizeof(float));
+}
+
+int monte_fdk_backproject(monte_ctx* c, const monte_geom* g, const float* filt, float* vol, int batch,
+                          int accumulate) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, true);
+  if (rc) return rc;
+  if (!filt || !vol || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g), nvx = vol_elems(*g);
+  float* pad = (float*)workspace(c, kSlotFwd, padded_elems(*g) * sizeof(float));
+  if (!pad) return fail(c, MONTE_ERR_NOMEM, "filtered stack allocation failed");
+  Stage st{c};
+  const float* din = (const float*)st.in(kSlotStageA, filt, np * batch * sizeof(float));
+  float* dout = (float*)st.out(kSlotStageB, vol, nvx * batch * sizeof(float), accumulate != 0);
+  if (st.code) return st.code;
+  const FilterParams fp = compute_filter_params(*g);
+  for (int b = 0; b < batch; ++b) {
+    MONTE_HIP(c, launch_pad_copy(din + b * np, pad, g->n_views, g->nu, g->nv, fp.pitch, fp.rows, c->stream));
+    if (g->preset == MONTE_PRESET_BP3D20)
+      MONTE_HIP(c, launch_fill_pads(pad, g->n_views, g->nu, g->nv, fp.pitch, fp.rows, 1, c->stream));
+    if ((rc = run_backproject_padded(c, *g, pad, dout + b * nvx, accumulate))) return rc;
+  }
+  return st.back(vol, dout, nvx * batch * sizeof(float));
+}
+
+int monte_fdk(monte_ctx* c, const monte_geom* g, const float* proj, float* vol, int batch) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, true);
+  if (rc) return rc;
+  if (!proj || !vol || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g), nvx = vol_elems(*g);
+  float* pad = (float*)workspace(c, kSlotFwd, padded_elems(*g) * sizeof(float));
+  if (!pad) return fail(c, MONTE_ERR_NOMEM, "filtered stack allocation failed");
+  Stage st{c};
+  const float* din = (const float*)st.in(kSlotStageA, proj, np * batch * sizeof(float));
+  float* dout = (float*)st.out(kSlotStageB, vol, nvx * batch * sizeof(float), false);
+  if (st.code) return st.code;
+  for (int b = 0; b < batch; ++b) {
+    if ((rc = run_filter_padded(c, *g, din + b * np, pad))) return rc;
+    if ((rc = run_backproject_padded(c, *g, pad, dout + b * nvx, 0))) return rc;
+  }
+  return st.back(vol, dout, nvx * batch * sizeof(float));
+}
+
+int monte_fdk_project(monte_ctx* c, const monte_geom* g, const float* vol, float* filt, int batch) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, true);
+  if (rc) return rc;
+  if (!vol || !filt || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g), nvx = vol_elems(*g);
+  float* pad = (float*)workspace(c, kSlotAdj, padded_elems(*g) * sizeof(float));
+  if (!pad) return fail(c, MONTE_ERR_NOMEM, "adjoint stack allocation failed");
+  Stage st{c};
+  const float* din = (const float*)st.in(kSlotStageA, vol, nvx * batch * sizeof(float));
+  float* dout = (float*)st.out(kSlotStageB, filt, np * batch * sizeof(float), false);
+  if (st.code) return st.code;
+  const FilterParams fp = compute_filter_params(*g);
+  for (int b = 0; b < batch; ++b) {
+    if ((rc = run_project_padded(c, *g, din + b * nvx, pad))) return rc;
+    MONTE_HIP(c, launch_unpad_copy(pad, dout + b * np, g->n_views, g->nu, g->nv, fp.pitch, fp.rows, c->stream));
+  }
+  return st.back(filt, dout, np * batch * sizeof(float));
+}
+
+int monte_fdk_filter_adjoint(monte_ctx* c, const monte_geom* g, const float* filt, float* proj, int batch) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, false);
+  if (rc) return rc;
+  if (!filt || !proj || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g);
+  float* pad = (float*)workspace(c, kSlotAdj, padded_elems(*g) * sizeof(float));
+  if (!pad) return fail(c, MONTE_ERR_NOMEM, "adjoint stack allocation failed");
+  Stage st{c};
+  const float* din = (const float*)st.in(kSlotStageA, filt, np * batch * sizeof(float));
+  float* dout = (float*)st.out(kSlotStageB, proj, np * batch * sizeof(float), false);
+  if (st.code) return st.code;
+  const FilterParams fp = compute_filter_params(*g);
+  for (int b = 0; b < batch; ++b) {
+    MONTE_HIP(c, launch_pad_copy(din + b * np, pad, g->n_views, g->nu, g->nv, fp.pitch, fp.rows, c->stream));
+    if ((rc = run_filter_T_padded(c, *g, pad, dout + b * np))) return rc;
+  }
+  return st.back(proj, dout, np * batch * sizeof(float));
+}
+
+int monte_fdk_adjoint(monte_ctx* c, const monte_geom* g, const float* dvol, float* dproj, int batch) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, true);
+  if (rc) return rc;
+  if (!dvol || !dproj || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g), nvx = vol_elems(*g);
+  float* pad = (float*)workspace(c, kSlotAdj, padded_elems(*g) * sizeof(float));
+  if (!pad) return fail(c, MONTE_ERR_NOMEM, "adjoint stack allocation failed");
+  Stage st{c};
+  const float* din = (const float*)st.in(kSlotStageA, dvol, nvx * batch * sizeof(float));
+  float* dout = (float*)st.out(kSlotStageB, dproj, np * batch * sizeof(float), false);
+  if (st.code) return st.code;
+  for (int b = 0; b < batch; ++b) {
+    if ((rc = run_project_padded(c, *g, din + b * nvx, pad))) return rc;
+    if ((rc = run_filter_T_padded(c, *g, pad, dout + b * np))) return rc;
+  }
+  return st.back(dproj, dout, np * batch * sizeof(float));
+}
+
+// ---------------------------------------------------------------------- MC
+int monte_mc_batch(monte_ctx* c, const monte_mc_job* jobs, int n_jobs, monte_mc_stats* stats) {
+  if (!c) return MONTE_ERR_INVALID;
+  if (!jobs || n_jobs < 1) return fail(c, MONTE_ERR_INVALID, "no jobs");
+  if (c->host_ptrs) return fail(c, MONTE_ERR_UNSUPPORTED, "monte_mc_batch takes device pointers only");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const int batch = 64 * c->tuning.mc_batch;
+  std::vector<HostJob> hj(n_jobs);
+  for (int i = 0; i < n_jobs; ++i) {
+    int rc = build_job(c, jobs[i].geom, jobs[i].phantom, jobs[i].cfg, hj[i], batch);
+    if (rc) return fail(c, rc, "job " + std::to_string(i) + ": " + c->err);
+    if (!jobs[i].primary || !jobs[i].scatter) return fail(c, MONTE_ERR_INVALID, "job tallies are null");
+    hj[i].d.primary = jobs[i].primary;
+    hj[i].d.scatter = jobs[i].scatter;
+  }
+  float ms = 0.f;
+  std::vector<McCounters> cnt;
+  int rc = run_mc(c, hj, batch, c->tuning.mc_block, c->tuning.mc_blocks_per_cu, &ms, &cnt);
+  if (rc) return rc;
+  if (stats)
+    for (int i = 0; i < n_jobs; ++i) fill_stats(&stats[i], cnt[i], ms);
+  return MONTE_OK;
+}
+
+int monte_mc_project(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const monte_mc_cfg* cfg,
+                     double* primary, double* scatter, monte_mc_stats* stats) {
+  if (!c) return MONTE_ERR_INVALID;
+  if (!g || !ph || !cfg || !primary || !scatter) return fail(c, MONTE_ERR_INVALID, "null argument");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g);
+  Stage st{c};
+  monte_phantom dph = *ph;
+  dph.labels = (const uint8_t*)st.in(kSlotStageA, ph->labels, (size_t)ph->nx * ph->ny * ph->nz);
+  double* dp = (double*)st.out(kSlotStageB, primary, np * sizeof(double), true);
+  double* ds = (double*)st.out(kSlotTuneA, scatter, np * sizeof(double), true);
+  if (st.code) return st.code;
+  const bool host = c->host_ptrs;
+  c->host_ptrs = false;
+  monte_mc_job job{g, &dph, cfg, dp, ds};
+  int rc = monte_mc_batch(c, &job, 1, stats);
+  c->host_ptrs = host;
+  if (rc) return rc;
+  if ((rc = st.back(primary, dp, np * sizeof(double)))) return rc;
+  return st.back(scatter, ds, np * sizeof(double));
+}
+
+int monte_primary_exact(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, double e_kev,
+                        const double* spectrum, float* lineint) {
+  if (!c) return MONTE_ERR_INVALID;
+  if (!g || !ph || !lineint) return fail(c, MONTE_ERR_INVALID, "null argument");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  monte_mc_cfg cfg{};
+  cfg.photons_per_view = 1;
+  cfg.e_kev = spectrum ? 60.0 : e_kev;
+  cfg.spectrum = nullptr;
+  HostJob hj;
+  Stage st{c};
+  monte_phantom dph = *ph;
+  dph.labels = (const uint8_t*)st.in(kSlotStageA, ph->labels, (size_t)ph->nx * ph->ny * ph->nz);
+  if (st.code) return st.code;
+  int rc = build_job(c, g, &dph, &cfg, hj, 64);
+  if (rc) return rc;
+  const size_t np = proj_elems(*g);
+  float* dout = (float*)st.out(kSlotStageB, lineint, np * sizeof(float), false);
+  if (st.code) return st.code;
+  // tables, view table and spectrum weights in the MC slot
+  size_t off = 0;
+  const size_t tab_off = off; off = align_up(off + sizeof(McTables), 256);
+  const size_t cs_off = off; off = align_up(off + sizeof(float2) * hj.view_cs.size(), 256);
+  const size_t w_off = off; off = align_up(off + sizeof(float) * kRows, 256);
+  char* base = (char*)workspace(c, kSlotMc, off);
+  if (!base) return fail(c, MONTE_ERR_NOMEM, "primary descriptor allocation failed");
+  std::vector<float> w(kRows, 0.f);
+  if (spectrum) {
+    double tot = 0;
+    for (int e = 1; e < kRows; ++e) {
+      if (!(spectrum[e] >= 0)) return fail(c, MONTE_ERR_INVALID, "negative spectrum weight");
+      tot += spectrum[e];
+    }
+    if (!(tot > 0)) return fail(c, MONTE_ERR_INVALID, "empty spectrum");
+    for (int e = 1; e < kRows; ++e) w[e] = (float)(spectrum[e] / tot);
+  }
+  MONTE_HIP(c, hipMemcpyAsync(base + tab_off, &hj.tables, sizeof(McTables), hipMemcpyHostToDevice, c->stream));
+  MONTE_HIP(c, hipMemcpyAsync(base + cs_off, hj.view_cs.data(), sizeof(float2) * hj.view_cs.size(),
+                              hipMemcpyHostToDevice, c->stream));
+  MONTE_HIP(c, hipMemcpyAsync(base + w_off, w.data(), sizeof(float) * kRows, hipMemcpyHostToDevice, c->stream));
+  hj.d.tables = (const McTables*)(base + tab_off);
+  hj.d.view_cs = (const float2*)(base + cs_off);
+  hj.d.view_begin = 0;
+  hj.d.n_views_sim = g->n_views;
+  MONTE_HIP(c, launch_primary_exact(hj.d, (const float*)(base + w_off), spectrum ? kRows : 0, dout, c->stream));
+  MONTE_HIP(c, hipStreamSynchronize(c->stream));  // host tables above must outlive the copies
+  return st.back(lineint, dout, np * sizeof(float));
+}
+
+int monte_counts_to_lineint(monte_ctx* c, const monte_geom* g, const double* counts, uint64_t photons_per_view,
+                            float* lineint) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, false);
+  if (rc) return rc;
+  if (!counts || !lineint || photons_per_view == 0) return fail(c, MONTE_ERR_INVALID, "null buffer or no photons");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g);
+  Stage st{c};
+  const double* din = (const double*)st.in(kSlotStageA, counts, np * sizeof(double));
+  float* dout = (float*)st.out(kSlotStageB, lineint, np * sizeof(float), false);
+  if (st.code) return st.code;
+  MONTE_HIP(c, launch_counts_to_lineint(din, dout, g->n_views, g->nu, g->nv, photons_per_view, c->stream));
+  return st.back(lineint, dout, np * sizeof(float));
+}
+
+// ----------------------------------------------------------------- phantom
+int monte_phantom_paint(uint8_t* labels, int nx, int ny, int nz, double vx, double vy, double vz,
+                        const monte_primitive* prims, int n_prims) {
+  if (!labels || nx < 1 || ny < 1 || nz < 1 || !(vx > 0) || !(vy > 0) || !(vz > 0) || n_prims < 0 ||
+      (n_prims > 0 && !prims))
+    return MONTE_ERR_INVALID;
+  for (int i = 0; i < n_prims; ++i)
+    if (prims[i].label < 0 || prims[i].label > 255 || prims[i].shape < 0 || prims[i].shape > 2 ||
+        !(prims[i].rx > 0) || !(prims[i].ry > 0) || !(prims[i].rz > 0))
+      return MONTE_ERR_INVALID;
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto work = [&](int z0, int z1) {
+    for (int z = z0; z < z1; ++z) {
+      const double Z = (z - 0.5 * (nz - 1)) * vz;
+      for (int y = 0; y < ny; ++y) {
+        const double Y = (y - 0.5 * (ny - 1)) * vy;
+        uint8_t* row = labels + ((size_t)z * ny + y) * nx;
+        for (int x = 0; x < nx; ++x) {
+          const double X = (x - 0.5 * (nx - 1)) * vx;
+          for (int i = 0; i < n_prims; ++i) {
+            const monte_primitive& p = prims[i];
+            const double a = (X - p.cx) / p.rx, b = (Y - p.cy) / p.ry, cc = (Z - p.cz) / p.rz;
+            bool in;
+            if (p.shape == MONTE_SHAPE_ELLIPSOID) in = a * a + b * b + cc * cc <= 1.0;
+            else if (p.shape == MONTE_SHAPE_CYLINDER_Z) in = a * a + b * b <= 1.0 && std::fabs(cc) <= 1.0;
+            else in = std::fabs(a) <= 1.0 && std::fabs(b) <= 1.0 && std::fabs(cc) <= 1.0;
+            if (in) row[x] = (uint8_t)p.label;
+          }
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  const int per = (nz + (int)nt - 1) / (int)nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const int z0 = (int)t * per, z1 = std::min(nz, z0 + per);
+    if (z0 < z1) th.emplace_back(work, z0, z1);
+  }
+  for (auto& t : th) t.join();
+  return MONTE_OK;
+}
+
+// ------------------------------------------------------------------ tuning
+int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phantom* ph, const monte_mc_cfg* cfg,
+                   monte_tuning* out) {
+  if (!c) return MONTE_ERR_INVALID;
+  if (c->host_ptrs) return fail(c, MONTE_ERR_UNSUPPORTED, "autotune takes device pointers only");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  monte_tuning best = c->tuning;
+  if (what & 1) {
+    if (!g || !ph || !cfg) return fail(c, MONTE_ERR_INVALID, "MC tuning needs geometry, phantom and config");
+    // a short run: 2 views, up to 2^21 photons each, into scratch tallies
+    monte_mc_cfg tc = *cfg;
+    tc.photons_per_view = std::min<uint64_t>(cfg->photons_per_view, 1u << 21);
+    int vb = cfg->view_begin, ve = cfg->view_end;
+    if (vb == 0 && ve == 0) ve = g->n_views;
+    tc.view_begin = vb;
+    tc.view_end = std::min(ve, vb + 2);
+    const size_t np = proj_elems(*g);
+    double* tally = (double*)workspace(c, kSlotTuneA, 2 * np * sizeof(double));
+    if (!tally) return fail(c, MONTE_ERR_NOMEM, "tuning tallies");
+    float best_ms = 1e30f;
+    const int blocks[] = {64, 128, 256};
+    const int batches[] = {4, 8, 16, 32};
+    const int bpcs[] = {2, 4, 8, 16};
+    for (int blk : blocks)
+      for (int bt : batches)
+        for (int bpc : bpcs) {
+          if (blk * bpc > 2048) continue;
+          HostJob hj;
+          int rc = build_job(c, g, ph, &tc, hj, 64 * bt);
+          if (rc) return rc;
+          hj.d.primary = tally;
+          hj.d.scatter = tally + np;
+          std::vector<HostJob> v(1, hj);
+          float ms = 0.f;
+          if ((rc = run_mc(c, v, 64 * bt, blk, bpc, &ms, nullptr))) return rc;
+          if (ms < best_ms) {
+            best_ms = ms;
+            best.mc_block = blk;
+            best.mc_batch = bt;
+            best.mc_blocks_per_cu = bpc;
+          }
+        }
+  }
+  if (what & 2) {
+    int rc = check_geom(c, g, true);
+    if (rc) return rc;
+    // time a slice of views on scratch buffers
+    monte_geom tg = *g;
+    const float* taps;
+    const BpView* views;
+    if ((rc = ensure_constants(c, tg, &taps, &views))) return rc;
+    const int nviews = std::min(g->n_views, 32);
+    float* pad = (float*)workspace(c, kSlotFwd, padded_elems(tg) * sizeof(float));
+    float* vol = (float*)workspace(c, kSlotTuneB, vol_elems(tg) * sizeof(float));
+    if (!pad || !vol) return fail(c, MONTE_ERR_NOMEM, "tuning buffers");
+    MONTE_HIP(c, launch_fill(pad, 0.f, padded_elems(tg), c->stream));
+    const BpGrid grid = compute_bp_grid(tg);
+    hipEvent_t e0, e1;
+    MONTE_HIP(c, hipEventCreate(&e0));
+    MONTE_HIP(c, hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    const int zcs[] = {4, 8, 16};
+    const int txs[] = {8, 16, 32, 64};
+    for (int zc : zcs)
+      for (int tx : txs) {
+        monte_tuning t = c->tuning;
+        t.bp_zchunk = zc;
+        t.bp_tile_x = tx;
+        t.bp_tile_y = 256 / tx;
+        MONTE_HIP(c, launch_backproject(pad, views, 0, nviews, grid, vol, 0, t, c->stream));  // warm
+        MONTE_HIP(c, hipEventRecord(e0, c->stream));
+        MONTE_HIP(c, launch_backproject(pad, views, 0, nviews, grid, vol, 0, t, c->stream));
+        MONTE_HIP(c, hipEventRecord(e1, c->stream));
+        MONTE_HIP(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        MONTE_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best.bp_zchunk = zc;
+          best.bp_tile_x = tx;
+          best.bp_tile_y = 256 / tx;
+        }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  c->tuning = best;
+  if (out) *out = best;
+  return MONTE_OK;
+}
+
+// ------------------------------------------------------------------ graphs
+int monte_fdk_graph(monte_ctx* c, const monte_geom* g, const float* proj, float* vol, int batch) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, true);
+  if (rc) return rc;
+  if (!proj || !vol || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
+  if (c->host_ptrs) return fail(c, MONTE_ERR_UNSUPPORTED, "graphs take device pointers only");
+  if (c->stream == nullptr) return fail(c, MONTE_ERR_INVALID, "graphs need a non-null stream");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  std::vector<double> key = geom_key(*g);
+  key.push_back((double)(uintptr_t)proj);
+  key.push_back((double)(uintptr_t)vol);
+  key.push_back((double)batch);
+  const monte_tuning& t = c->tuning;
+  for (double v : {(double)t.bp_zchunk, (double)t.bp_tile_x, (double)t.bp_view_chunk}) key.push_back(v);
+  // everything the captured launches touch must exist before capture
+  const float* taps;
+  const BpView* views;
+  if ((rc = ensure_constants(c, *g, &taps, &views))) return rc;
+  float* pad = (float*)workspace(c, kSlotFwd, padded_elems(*g) * sizeof(float));
+  if (!pad) return fail(c, MONTE_ERR_NOMEM, "filtered stack allocation failed");
+  key.push_back((double)(uintptr_t)pad);
+  key.push_back((double)(uintptr_t)c->ws[kSlotConst]);
+  if (key != c->graph_key || !c->graph_exec) {
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    c->graph_key.clear();
+    MONTE_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const size_t np = proj_elems(*g), nvx = vol_elems(*g);
+    int crc = MONTE_OK;
+    for (int b = 0; b < batch && !crc; ++b) {
+      crc = run_filter_padded(c, *g, proj + b * np, pad);
+      if (!crc) crc = run_backproject_padded(c, *g, pad, vol + b * nvx, 0);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    if (crc) { if (graph) (void)hipGraphDestroy(graph); return crc; }
+    if (ce != hipSuccess) return fail(c, MONTE_ERR_HIP, std::string("end capture: ") + hipGetErrorString(ce));
+    c->graph = graph;
+    MONTE_HIP(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    c->graph_key = key;
+  }
+  MONTE_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+  return MONTE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// Internal declarations shared by the host C++ and the gfx950 kernels of
+// libmonte.so.  Not part of the public ABI (include/monte.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/monte.h"
+
+namespace monte {
+
+constexpr int kRows = MONTE_TABLE_ROWS;
+constexpr int kMaxLabels = MONTE_MAX_LABELS;
+constexpr int kPadLo = 1;  // zero guard cells before u / v in padded filtered projections
+constexpr int kPadHi = 3;  // and after (covers the +1 bilinear tap at the window edge)
+
+// ------------------------------------------------------------------ FDK
+// Per-view backprojection coefficients (computed on the host in FP64).  For a
+// voxel centre (X,Y,Z):
+//   U  = cx X + cy Y + c0                      distance along the central ray
+//   su = su0 + suk (ax X + ay Y) / U           sample coordinate along u
+//   sv = sv0 + svk Z / U                       sample coordinate along v
+//   Dd = d0 + dx X + dy Y - sgn(gx X + gy Y) |hx X + hy Y|
+//   w  = fac (wn / Dd)^2                       distance weight x angular step
+// Standard preset: Dd = U, wn = sod, fac = dangle.  bp3d20 preset: Dd = 60 - d
+// with d the signed point-line distance of recon/bp3d20_325.cpp:145-153.
+struct BpView {
+  float cx, cy, c0;
+  float ax, ay;
+  float su0, suk;
+  float sv0, svk;
+  float wn, d0, dx, dy;
+  float gx, gy, hx, hy;
+  float fac;
+  float pad0, pad1;
+};
+static_assert(sizeof(BpView) == 80, "BpView layout");
+
+// Geometry-wide constants of the backprojector.
+struct BpGrid {
+  int nx, ny, nz, nu, nv;
+  int pitch;          // padded row pitch of filtered projections (nu + kPadLo + kPadHi)
+  int rows;           // padded rows (nv + kPadLo + kPadHi)
+  float gx0, gsx;     // X = gx0 + x * gsx
+  float gy0, gsy;
+  float gz0, gsz;
+  float lo_u, hi_u;   // valid sample window along u (inclusive)
+  float lo_v, hi_v;
+  int x0, x1;         // x range reconstructed (bp3d20 ROI)
+  int mask;           // 1: sphere mask in voxel index space
+  float mask_r2;
+  int mcx, mcy, mcz;  // mask centre (voxel index)
+};
+
+// Filter constants: weight(r,c) = D / sqrt(D^2 + (a_u + b_u r)^2 + (a_v + b_v c)^2),
+// out = cs * sum_r weight * p * h[r - b].
+struct FilterParams {
+  int nu, nv, n_views;
+  int pitch, rows;    // padded output layout
+  float D, a_u, b_u, a_v, b_v;
+  float cs;
+};
+
+void compute_bp_views(const monte_geom& g, std::vector<BpView>& out);
+BpGrid compute_bp_grid(const monte_geom& g);
+FilterParams compute_filter_params(const monte_geom& g);
+void compute_filter_taps(const monte_geom& g, std::vector<float>& h);  // h[0..2nu-2], centre nu-1
+
+// ------------------------------------------------------------------- MC
+// Device attenuation tables of one job, per label and keV row:
+//   m[l][e] = (mu, photo/total, (photo+coherent)/total, 0), mu in cm^-1
+//   mumax[e] = max_l mu  (the Woodcock majorant)
+struct McTables {
+  float4 m[kMaxLabels][kRows];
+  float mumax[kRows];
+  float pad[3];
+};
+
+// Device job descriptor of the persistent transport kernel.
+struct McJob {
+  const uint8_t* labels;
+  const McTables* tables;
+  const float* spectrum_cdf;     // [kRows] or null (mono)
+  const float2* view_cs;         // cos, sin per view
+  double* primary;
+  double* scatter;
+  int nx, ny, nz, nu, nv;
+  int view_begin, n_views_sim;   // simulated views (indices into the tally arrays)
+  float vx, vy, vz;
+  float bx0, by0, bz0, bx1, by1, bz1;  // phantom box
+  float inv_vx, inv_vy, inv_vz;
+  float sod, sdd;
+  float un0, us, vn0, vs;        // u_idx = un0 + us * y', v_idx = vn0 + vs * z'
+  float u_step, v_step;          // 1/us, 1/vs (signed pitch)
+  float e_kev;
+  int max_int, coherent;
+  uint32_t key0, key1, job_id;
+  uint32_t photons_per_view;
+  uint32_t batches_per_view;     // ceil(photons_per_view / batch)
+  uint64_t unit_begin;           // first global work unit of this job
+  uint64_t unit_count;
+};
+
+struct McCounters {  // device-side statistics, one per job
+  unsigned long long histories, steps, interactions, photo, coherent, compton;
+  unsigned long long tallied_primary, tallied_scatter, missed, capped;
+};
+
+}  // namespace monte
+
+// ------------------------------------------------------------------ context
+struct monte_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  bool host_ptrs = false;
+  std::string err;
+  int n_cu = 256;
+
+  // attenuation tables (host copy, FP64): [n_tables][4][201]
+  std::vector<double> tables;
+  int n_tables = 0;
+
+  monte_tuning tuning{};
+
+  // growable device workspaces:
+  //   0 forward filtered stack (padded)   1 adjoint filtered stack (padded)
+  //   2 MC descriptors                    3 FDK constants (taps + per-view coefficients)
+  //   4..7 staging for host-pointer mode and tuning scratch
+  static constexpr int kSlots = 8;
+  void* ws[kSlots] = {};
+  size_t ws_size[kSlots] = {};
+
+  // FDK constants cached by geometry
+  std::vector<double> const_key;
+  size_t taps_off = 0, views_off = 0;
+
+  // hipGraph cache for monte_fdk_graph
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<double> graph_key;
+};
+
+namespace monte {
+
+// error helpers
+int fail(monte_ctx* c, int code, const std::string& msg);
+#define MONTE_HIP(ctx, expr)                                                              \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ::monte::fail((ctx), MONTE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// workspace slot i grown to at least bytes; returns nullptr on failure
+void* workspace(monte_ctx* c, int slot, size_t bytes);
+
+// ---------------------------------------------------------- kernel launches
+// FDK (fdk.hip)
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
+hipError_t launch_fdk_filter(const float* proj, float* filt_pad, const float* taps, const FilterParams& fp,
+                             hipStream_t s);
+hipError_t launch_pad_copy(const float* filt, float* filt_pad, int n_views, int nu, int nv, int pitch, int rows,
+                           hipStream_t s);
+hipError_t launch_unpad_copy(const float* filt_pad, float* filt, int n_views, int nu, int nv, int pitch, int rows,
+                             hipStream_t s);
+hipError_t launch_fill_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, int wrap, hipStream_t s);
+hipError_t launch_fold_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, hipStream_t s);
+hipError_t launch_backproject(const float* filt_pad, const BpView* views, int view0, int nviews_chunk,
+                              const BpGrid& grid, float* vol, int accumulate, const monte_tuning& t, hipStream_t s);
+hipError_t launch_project(const float* vol, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
+                          float* filt_pad, const monte_tuning& t, hipStream_t s);
+hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* taps, const FilterParams& fp,
+                               hipStream_t s);
+
+// MC (mc.hip)
+hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, unsigned int* queue,
+                               McCounters* counters, int batch, int block, int grid, hipStream_t s);
+hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs, hipStream_t s);
+hipError_t launch_primary_exact(const McJob& job, const float* spec_w, int n_spec, float* out, hipStream_t s);
+hipError_t launch_counts_to_lineint(const double* counts, float* out, int n_views, int nu, int nv,
+                                    uint64_t photons_per_view, hipStream_t s);
+
+}  // namespace monte

@@ -1,0 +1,126 @@
+"""Monte Carlo transport and the exact primary estimator on the GPU.
+
+* Primary-only projection (scatter off) vs exact Beer-Lambert line integrals
+  from the FP64 Siddon oracle: 1e-5 relative.
+* Full MC vs the FP64 CPU oracle at equal photon count.  Both draw every
+  history from the same Philox stream in the same order, so almost all
+  histories coincide; the test asserts per-pixel agreement within 3 sigma of
+  the Poisson bound, reports the chi-square statistic and the fraction of
+  identical pixels.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, oracle_geom, oracle_phantom
+
+pytestmark = pytest.mark.gpu
+
+
+def _tables(oracle):
+    return oracle.tables(np.stack([oracle.read_xcom_csv(os.path.join(ROOT, "monte_amd", "data", f))[0]
+                                   for f in ("xcom2.csv", "Ca.csv")]))
+
+
+@pytest.mark.parametrize("mode", ["mono60", "mono30", "spectrum"])
+def test_primary_exact_matches_siddon(ctx, oracle, mode):
+    from monte_amd import geometry as G
+    from monte_amd import phantoms, tables
+    from monte_amd.mc import primary_exact
+
+    geom = G.small(n_views=8)
+    ph = phantoms.small_phantom()
+    spec = tables.kramers_spectrum(120.0, 2.5) if mode == "spectrum" else None
+    e = 30.0 if mode == "mono30" else 60.0
+    got = primary_exact(ph, geom, e_kev=e, spectrum=spec, ctx=ctx).cpu().numpy()
+    ref = oracle.primary_lineint(oracle_geom(oracle, geom), oracle_phantom(oracle, ph), _tables(oracle), e, spec)
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    print(mode, "max rel", err.max(), "max L", ref.max())
+    assert err.max() < 1e-5
+
+
+def _mc_pair(ctx, oracle, geom, ph, photons, **kw):
+    from monte_amd.mc import McSettings, mc_project
+
+    s = McSettings(photons_per_view=photons, **kw)
+    prim, scat, st = mc_project(ph, geom, s, ctx=ctx)
+    spec_cdf = None
+    if s.spectrum is not None:
+        from monte_amd.tables import spectrum_cdf_f32
+
+        spec_cdf = spectrum_cdf_f32(s.spectrum)
+    p_ref, s_ref, st_ref = oracle.mc_run(oracle_geom(oracle, geom), oracle_phantom(oracle, ph), _tables(oracle),
+                                         photons, e_kev=s.e_kev, spectrum_cdf=spec_cdf,
+                                         max_interactions=s.max_interactions, coherent_mode=s.coherent,
+                                         seed=s.seed, job=s.job)
+    return prim.cpu().numpy(), scat.cpu().numpy(), st, p_ref, s_ref, st_ref
+
+
+@pytest.mark.parametrize("coherent", [0, 1])
+def test_mc_matches_oracle_small(ctx, oracle, coherent):
+    from monte_amd import geometry as G
+    from monte_amd import phantoms
+
+    geom = G.small(n_views=4)
+    ph = phantoms.small_phantom()
+    N = 200_000
+    prim, scat, st, p_ref, s_ref, st_ref = _mc_pair(ctx, oracle, geom, ph, N, e_kev=60.0, coherent=coherent)
+    print("gpu", st)
+    print("cpu", st_ref)
+    assert st["histories"] == st_ref["histories"] == N * geom.n_views
+    for a, b in ((prim, p_ref), (scat, s_ref)):
+        assert a.sum() > 0
+        same = np.mean(a == b)
+        var = np.maximum(a + b, 1.0)
+        z = (a - b) / np.sqrt(var)
+        chi2 = float(np.sum((a - b) ** 2 / var))
+        print(f"identical pixels {same:.4f}  max|z| {np.abs(z).max():.2f}  chi2/dof {chi2 / a.size:.4f}")
+        assert np.abs(z).max() < 3.0
+        assert same > 0.95
+    # totals of the per-history statistics agree to the same few divergent histories
+    for k in ("tallied_primary", "tallied_scatter", "photo", "compton", "coherent"):
+        assert abs(st[k] - st_ref[k]) <= max(10, 1e-3 * st_ref[k]), k
+
+
+def test_mc_primary_matches_beer_lambert(ctx, oracle):
+    """Primary counts are Binomial(N0, exp(-L)) with L the exact line integral."""
+    from monte_amd import geometry as G
+    from monte_amd import phantoms
+    from monte_amd.mc import McSettings, mc_project
+
+    geom = G.small(n_views=2)
+    ph = phantoms.small_phantom()
+    npix = geom.nu * geom.nv
+    N = 200 * npix
+    prim, _, st = mc_project(ph, geom, McSettings(photons_per_view=N, e_kev=60.0), ctx=ctx)
+    L = oracle.primary_lineint(oracle_geom(oracle, geom), oracle_phantom(oracle, ph), _tables(oracle), 60.0)
+    p = np.exp(-L)
+    n0 = 200
+    mean = n0 * p
+    sd = np.sqrt(np.maximum(n0 * p * (1 - p), 1e-12))
+    z = (prim.cpu().numpy() - mean) / sd
+    inside = p < 0.999
+    chi2 = float(np.sum(z[inside] ** 2)) / inside.sum()
+    print("chi2/dof", chi2, "max|z|", np.abs(z[inside]).max())
+    assert 0.9 < chi2 < 1.1
+    # photons through air only are never absorbed
+    assert np.all(prim.cpu().numpy()[~inside] == n0)
+
+
+def test_counts_to_lineint(ctx):
+    import torch
+
+    from monte_amd import geometry as G
+    from monte_amd.mc import counts_to_lineint
+
+    geom = G.small(n_views=2)
+    N = 61 * geom.nu * geom.nv + 1000  # uneven: the first 1000 pixels get 62 photons
+    rng = np.random.default_rng(5)
+    counts = rng.integers(0, 70, size=geom.proj_shape).astype(np.float64)
+    got = counts_to_lineint(torch.from_numpy(counts).cuda(), geom, N, ctx).cpu().numpy()
+    n0 = np.full(geom.nu * geom.nv, 61.0)
+    n0[:1000] = 62.0
+    n0 = n0.reshape(geom.nu, geom.nv)[None]
+    ref = np.log(n0) - np.log(np.clip(counts, 1.0, n0))
+    assert np.allclose(got, ref, rtol=1e-6, atol=1e-6)
