@@ -1,0 +1,227 @@
+"""Benchmark of the two hot paths on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--photons P] [--views V] [--cpu-seconds S]
+
+Workload (full CBCT): 325^3 label phantom at 0.5 mm (water ellipsoid 300 x 250 x
+300 voxels, 12 seeded Ca inserts r in [3,15] voxels), 512 x 512 detector at
+1.2 mm, SOD 600 / SDD 1200 mm, 360 views, 120 kVp Kramers spectrum with
+2.5 mm Al in 1 keV bins, 1e8 photons per view, FP32 tracking with FP64
+tallies; FDK (Shepp-Logan) of the resulting line integrals to 325^3 FP32;
+and a differentiable-FDK step (forward FDK + adjoint through
+torch.autograd) on a batch of 8 stacks of the same geometry.
+
+One step = MC projection of all views + counts->line integrals + FDK + the
+autograd forward/backward.  Inputs (phantom, stacks) are resident in HBM
+before timing.  Prints one JSON line; `value` is MC histories per second.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_LANE_OPS = 256 * 64 * 2.4e9  # MI355X FP32 VALU: 256 CUs x 64 lanes x 2.4 GHz, one op per lane per clock
+PEAK_HBM = 8.0e12
+BP_OPS = 18       # FP32 VALU ops per backprojection voxel-update (algorithmic)
+MC_OPS_STEP, MC_OPS_INT, MC_OPS_HIST = 40, 150, 60  # per Woodcock step, per interaction, per history
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons", type=float, default=1e8, help="photons per view")
+    ap.add_argument("--views", type=int, default=360)
+    ap.add_argument("--batch", type=int, default=8, help="stacks in the differentiable-FDK step")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline budget (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=1)
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from monte_amd import Context, geometry, phantoms, tables
+    from monte_amd.fdk import fdk, fdk_autograd
+    from monte_amd.mc import McSettings, autotune, counts_to_lineint, device_labels, mc_project
+
+    dev = torch.device("cuda", 0)
+    ctx = Context(0)
+    geom = geometry.full(n_views=a.views)
+    ph = phantoms.full_phantom()
+    labels = device_labels(ph, dev)
+    spec = tables.kramers_spectrum(120.0, 2.5)
+    N = int(a.photons)
+    settings = McSettings(photons_per_view=N, spectrum=spec, coherent=1, seed=20261202)
+    primary = torch.zeros(geom.proj_shape, dtype=torch.float64, device=dev)
+    scatter = torch.zeros_like(primary)
+    rng = np.random.default_rng(20261202)
+    # differentiable-FDK input: noisy stacks (Poisson at 1e5 counts per pixel) from the exact primary
+    from monte_amd.mc import primary_exact
+
+    L = primary_exact(ph, geom, spectrum=spec, labels_dev=labels, ctx=ctx)
+    stacks = []
+    for b in range(a.batch):
+        counts = torch.poisson(1e5 * torch.exp(-L.double()), generator=torch.Generator(device=dev).manual_seed(b))
+        stacks.append((np.log(1e5) - torch.log(torch.clamp(counts, min=1.0))).float())
+    stacks = torch.stack(stacks).contiguous()
+    del L
+    tuning = {}
+    if a.autotune:
+        tuning = autotune(ph, geom, settings, what=3, labels_dev=labels, ctx=ctx)
+    torch.cuda.synchronize()
+
+    def step():
+        rec = {}
+        primary.zero_()
+        scatter.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _, _, st = mc_project(ph, geom, settings, labels_dev=labels, primary=primary, scatter=scatter, ctx=ctx)
+        e1.record()
+        lineint = counts_to_lineint(primary + scatter, geom, N, ctx)
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record()
+        vol = fdk(lineint, geom, ctx)
+        f1.record()
+        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        x = stacks.clone().requires_grad_(True)
+        g0.record()
+        v = fdk_autograd(x, geom, ctx)
+        v.backward(v.detach())  # gradient of 0.5 ||v||^2
+        g1.record()
+        h0, h1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        h0.record()
+        fdk(stacks, geom, ctx)
+        h1.record()
+        torch.cuda.synchronize()
+        rec["mc_ms"] = e0.elapsed_time(e1)
+        rec["mc_kernel_ms"] = st["kernel_ms"]
+        rec["mc_stats"] = st
+        rec["fdk_ms"] = f0.elapsed_time(f1)
+        rec["fwd_adj_ms"] = g0.elapsed_time(g1)
+        rec["fdk_batch_ms"] = h0.elapsed_time(h1)
+        rec["vol_mean"] = float(vol.mean())
+        return rec
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = [step() for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    hist = N * a.views
+    mc_s = np.mean([r["mc_ms"] for r in recs]) / 1e3
+    mc_k = np.mean([r["mc_kernel_ms"] for r in recs]) / 1e3
+    st = recs[-1]["mc_stats"]
+    S = st["steps"] / st["histories"]
+    Cn = st["interactions"] / st["histories"]
+    w_mc = MC_OPS_STEP * S + MC_OPS_INT * Cn + MC_OPS_HIST
+    fdk_s = np.mean([r["fdk_ms"] for r in recs]) / 1e3
+    updates = geom.nx * geom.ny * geom.nz * geom.n_views
+    fwd_adj = np.mean([r["fwd_adj_ms"] for r in recs])
+    fdk_batch = np.mean([r["fdk_batch_ms"] for r in recs])
+    out = {
+        "metric": "MC histories/s; FDK GUPS; fwd+adjoint ms at 325³, vs plain same GPU",
+        "value": hist / mc_s,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 tracking, f64 tallies",
+        "data": "synthetic (seeded phantom, generated 120 kVp spectrum)",
+        "config": {"workload": "full CBCT 325^3 / 512^2 / %d views / %.0e photons per view / FDK Shepp-Logan; "
+                               "diff-FDK batch %d" % (a.views, N, a.batch),
+                   "phantom": "325^3 u8 at 0.5 mm", "detector": "512x512 at 1.2 mm", "views": a.views,
+                   "photons_per_view": N, "spectrum": "120 kVp Kramers + 2.5 mm Al", "parallelism": "single GPU"},
+        "fdk_gups": updates / fdk_s / 1e9,
+        "fdk_ms": fdk_s * 1e3,
+        "fwd_adjoint_ms": fwd_adj,
+        "fdk_batch_ms": fdk_batch,
+        "fwd_adjoint_over_fdk": fwd_adj / fdk_batch,
+        "mc_steps_per_history": S,
+        "mc_interactions_per_history": Cn,
+        "roofline": {
+            "bound": "valu", "kernel": "mc_transport_kernel",
+            "achieved": w_mc * hist / mc_k / 1e12, "peak": PEAK_LANE_OPS / 1e12, "unit": "Tlane-op/s",
+            "frac": w_mc * hist / mc_k / PEAK_LANE_OPS, "traffic": None,
+            "work_per_unit": "W = 40 S + 150 C + 60 lane-ops per history (S steps, C interactions)",
+        },
+        "roofline_fdk_backproject": {
+            "bound": "valu", "achieved": BP_OPS * updates / fdk_s / 1e12, "peak": PEAK_LANE_OPS / 1e12,
+            "unit": "Tlane-op/s", "frac": BP_OPS * updates / fdk_s / PEAK_LANE_OPS,
+            "note": "whole FDK time (filter + backprojection) against the backprojection roof",
+        },
+        "tuning": tuning,
+    }
+    if a.cpu_seconds > 0:
+        out["cpu_baseline"], out["cpu_baseline_fdk"] = cpu_baselines(a, geom, ph, spec, hist / mc_s, updates / fdk_s)
+    print(json.dumps(out))
+
+
+def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
+    """Oracle (FP64 C restatement) on the host cores, on a bounded sample."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    from monte_amd import tables
+
+    nthreads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    og = O.geom(**geom.as_dict())
+    oph = O.phantom(ph.labels, *ph.voxel, materials=ph.materials)
+    tb = O.tables(tables.reference_tables())
+    cdf = tables.spectrum_cdf_f32(spec)
+    budget = a.cpu_seconds / 2
+    # MC: grow the photon sample until it takes a measurable fraction of the budget
+    n = 20000
+    while True:
+        t0 = time.perf_counter()
+        O.mc_run(og, oph, tb, n, spectrum_cdf=cdf, coherent_mode=1, views=(0, 2), nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        if dt > budget / 4 or n > 2e8:
+            break
+        n = int(n * max(2.0, min(8.0, budget / 4 / max(dt, 1e-3))))
+    mc = {"value": 2 * n / dt, "unit": "histories/s", "cores": nthreads, "kind": "port",
+          "sample": f"oracle MC (FP64, same physics and streams) on 2 views x {n} photons of the workload, "
+                    f"{dt:.1f} s", "gpu_over_cpu": gpu_hps / (2 * n / dt)}
+    # FDK: full filter + backprojection of a subset of views onto the full volume
+    nv = 2
+    while True:
+        g = O.geom(**{**geom.as_dict(), "n_views": nv, "dangle": geom.c().dangle})
+        proj = np.random.default_rng(0).standard_normal((nv, geom.nu, geom.nv)).astype(np.float32)
+        t0 = time.perf_counter()
+        O.fdk_std(g, proj, nthreads)
+        dt = time.perf_counter() - t0
+        if dt > budget / 4 or nv >= geom.n_views:
+            break
+        nv = min(geom.n_views, int(nv * max(2.0, min(8.0, budget / 4 / max(dt, 1e-3)))))
+    ups = nv * geom.nx * geom.ny * geom.nz / dt
+    fdk = {"value": ups / 1e9, "unit": "GUPS", "cores": nthreads, "kind": "port",
+           "sample": f"oracle FDK (FP64) of {nv} views onto the full 325^3 volume, {dt:.1f} s",
+           "gpu_over_cpu": gpu_ups / ups}
+    return mc, fdk
+
+
+if __name__ == "__main__":
+    main()

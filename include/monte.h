@@ -14,8 +14,9 @@
  *   - Data pointers are DEVICE pointers (hipMalloc or torch tensors) unless the
  *     context was switched to host pointers with monte_set_pointer_mode(); in
  *     host mode every call stages through device buffers it allocates itself.
- *   - Calls are ordered on the context's stream (monte_set_stream; default: the
- *     context's own non-blocking stream).  Use one context per host thread.
+ *   - Calls are ordered on the context's stream (monte_set_stream; a new
+ *     context starts on its own non-blocking stream).  Use one context per
+ *     host thread.
  *   - Units: cm, keV, cm^-1.  All layouts are C order, slowest index first:
  *       projections / tallies / line integrals  [view][u][v]   (v fastest)
  *       filtered projections                    [view][v][u]   (u fastest)
@@ -146,7 +147,8 @@ typedef struct monte_tuning {
 int monte_create(monte_ctx **ctx, int device);
 void monte_destroy(monte_ctx *ctx);
 /* stream: a hipStream_t (e.g. torch.cuda.current_stream().cuda_stream); NULL
- * returns to the context's own stream */
+ * selects the device's default (null) stream.  A new context starts on a
+ * non-blocking stream of its own. */
 int monte_set_stream(monte_ctx *ctx, void *stream);
 int monte_set_pointer_mode(monte_ctx *ctx, int host_pointers);
 int monte_synchronize(monte_ctx *ctx);

@@ -444,7 +444,7 @@ void monte_destroy(monte_ctx* c) {
 
 int monte_set_stream(monte_ctx* c, void* stream) {
   if (!c) return MONTE_ERR_INVALID;
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  c->stream = (hipStream_t)stream;  // NULL: the device's default (null) stream
   return MONTE_OK;
 }
 
@@ -714,10 +714,10 @@ int monte_primary_exact(monte_ctx* c, const monte_geom* g, const monte_phantom* 
   const size_t np = proj_elems(*g);
   float* dout = (float*)st.out(kSlotStageB, lineint, np * sizeof(float), false);
   if (st.code) return st.code;
-  // tables, view table and spectrum weights in the MC slot
+  // tables, FP64 view table and spectrum weights in the MC slot
   size_t off = 0;
   const size_t tab_off = off; off = align_up(off + sizeof(McTables), 256);
-  const size_t cs_off = off; off = align_up(off + sizeof(float2) * hj.view_cs.size(), 256);
+  const size_t cs_off = off; off = align_up(off + sizeof(double) * 2 * g->n_views, 256);
   const size_t w_off = off; off = align_up(off + sizeof(float) * kRows, 256);
   char* base = (char*)workspace(c, kSlotMc, off);
   if (!base) return fail(c, MONTE_ERR_NOMEM, "primary descriptor allocation failed");
@@ -731,15 +731,36 @@ int monte_primary_exact(monte_ctx* c, const monte_geom* g, const monte_phantom* 
     if (!(tot > 0)) return fail(c, MONTE_ERR_INVALID, "empty spectrum");
     for (int e = 1; e < kRows; ++e) w[e] = (float)(spectrum[e] / tot);
   }
+  std::vector<double> cs(2 * g->n_views);
+  for (int k = 0; k < g->n_views; ++k) {
+    const double b = g->preset == MONTE_PRESET_BP3D20 ? 3.14159265358979323846 * (k * g->step_deg) / 180.0
+                                                      : g->angle0 + k * g->dangle;
+    cs[2 * k] = std::cos(b);
+    cs[2 * k + 1] = std::sin(b);
+  }
+  PrimaryGeom P{};
+  P.sod = g->sod; P.sdd = g->sdd;
+  if (g->preset == MONTE_PRESET_BP3D20) {
+    P.un0 = 0.5 * g->nu; P.u_step = -g->du; P.vn0 = 0.5 * g->nv; P.v_step = -g->dv;
+  } else {
+    P.un0 = 0.5 * g->nu - g->u_off / g->du; P.u_step = g->du;
+    P.vn0 = 0.5 * g->nv - g->v_off / g->dv; P.v_step = g->dv;
+  }
+  const double vox[3] = {ph->vx, ph->vy, ph->vz}, org[3] = {ph->ox, ph->oy, ph->oz};
+  const int n[3] = {ph->nx, ph->ny, ph->nz};
+  for (int a = 0; a < 3; ++a) {
+    P.vox[a] = vox[a];
+    P.n[a] = n[a];
+    P.b0[a] = org[a] - 0.5 * n[a] * vox[a];
+    P.b1[a] = org[a] + 0.5 * n[a] * vox[a];
+  }
+  P.nu = g->nu; P.nv = g->nv; P.n_views = g->n_views;
   MONTE_HIP(c, hipMemcpyAsync(base + tab_off, &hj.tables, sizeof(McTables), hipMemcpyHostToDevice, c->stream));
-  MONTE_HIP(c, hipMemcpyAsync(base + cs_off, hj.view_cs.data(), sizeof(float2) * hj.view_cs.size(),
-                              hipMemcpyHostToDevice, c->stream));
+  MONTE_HIP(c, hipMemcpyAsync(base + cs_off, cs.data(), sizeof(double) * cs.size(), hipMemcpyHostToDevice, c->stream));
   MONTE_HIP(c, hipMemcpyAsync(base + w_off, w.data(), sizeof(float) * kRows, hipMemcpyHostToDevice, c->stream));
-  hj.d.tables = (const McTables*)(base + tab_off);
-  hj.d.view_cs = (const float2*)(base + cs_off);
-  hj.d.view_begin = 0;
-  hj.d.n_views_sim = g->n_views;
-  MONTE_HIP(c, launch_primary_exact(hj.d, (const float*)(base + w_off), spectrum ? kRows : 0, dout, c->stream));
+  MONTE_HIP(c, launch_primary_exact(P, (const double*)(base + cs_off), dph.labels, (const McTables*)(base + tab_off),
+                                    (float)cfg.e_kev, (const float*)(base + w_off), spectrum ? 1 : 0, dout,
+                                    c->stream));
   MONTE_HIP(c, hipStreamSynchronize(c->stream));  // host tables above must outlive the copies
   return st.back(lineint, dout, np * sizeof(float));
 }
@@ -900,7 +921,7 @@ int monte_fdk_graph(monte_ctx* c, const monte_geom* g, const float* proj, float*
   if (rc) return rc;
   if (!proj || !vol || batch < 1) return fail(c, MONTE_ERR_INVALID, "null buffer or batch < 1");
   if (c->host_ptrs) return fail(c, MONTE_ERR_UNSUPPORTED, "graphs take device pointers only");
-  if (c->stream == nullptr) return fail(c, MONTE_ERR_INVALID, "graphs need a non-null stream");
+  if (c->stream == nullptr) return fail(c, MONTE_ERR_INVALID, "graph capture needs a created stream, not the null stream");
   MONTE_HIP(c, hipSetDevice(c->device));
   std::vector<double> key = geom_key(*g);
   key.push_back((double)(uintptr_t)proj);

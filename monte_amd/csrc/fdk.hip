@@ -326,31 +326,98 @@ __device__ inline int xcd_tile(int b, int nblocks) {
   return xcd * q + (xcd < r ? xcd : r) + j;
 }
 
-// Per-column quantities of one view (shared by the forward and transposed
-// kernels so both evaluate bit-identical weights).
+// Per-column quantities of one view, shared by the backprojector and its
+// transpose so both evaluate bit-identical weights.
 struct ColumnView {
-  float su;     // sample coordinate along u
-  float svk;    // d(sv)/dZ
-  float sv0;
+  int iu;       // u pixel index of the left tap (sample window clamped)
+  float fu;     // u fraction
+  int ivb;      // integer part of sv at the base voxel zb
+  float fvb;    // its fraction
+  float dsv;    // sv increment per voxel step along z
+  float flo, fhi;  // v window relative to ivb
   float w;      // distance weight x angular factor
   bool ok;      // su inside the window
 };
 
-__device__ inline ColumnView column_view(const BpView& c, float X, float Y, const BpGrid& g) {
+__device__ inline ColumnView column_view(const BpView& c, double X, double Y, float Xf, float Yf, double Zb,
+                                         const BpGrid& g) {
   ColumnView r;
-  const float U = fmaf(c.cx, X, fmaf(c.cy, Y, c.c0));
-  const float inv = 1.0f / U;
-  const float nnum = fmaf(c.ax, X, c.ay * Y);
-  r.su = fmaf(c.suk * nnum, inv, c.su0);
-  r.svk = c.svk * inv;
-  r.sv0 = c.sv0;
-  const float gp = fmaf(c.gx, X, c.gy * Y);
-  const float hp = fabsf(fmaf(c.hx, X, c.hy * Y));
-  const float dd = fmaf(c.dx, X, fmaf(c.dy, Y, c.d0)) - (gp < 0.f ? -hp : hp);
+  const double U = fma(c.cx, X, fma(c.cy, Y, c.c0));
+  const double inv = 1.0 / U;
+  double su = fma(c.suk * fma(c.ax, X, c.ay * Y), inv, c.su0);
+  r.ok = su >= g.lo_u && su <= g.hi_u;
+  su = fmin(fmax(su, g.lo_u), g.hi_u);
+  const double fu0 = floor(su);
+  r.iu = (int)fu0;
+  r.fu = (float)(su - fu0);
+  const double k = c.svk * inv;
+  double sv = fma(k, Zb, c.sv0);
+  sv = fmin(fmax(sv, -1.0e6), 1.0e6);
+  const double fv0 = floor(sv);
+  r.ivb = (int)fv0;
+  r.fvb = (float)(sv - fv0);
+  r.dsv = (float)(k * g.gsz);
+  r.flo = (float)(g.lo_v - fv0);
+  r.fhi = (float)(g.hi_v - fv0);
+  // sign test in unfused FP64 (as recon/bp3d20_325.cpp:145,151 evaluates it)
+  const double gp = __dadd_rn(__dmul_rn(X, c.gx), __dmul_rn(Y, c.gy));
+  const float hp = fabsf(fmaf(c.hx, Xf, c.hy * Yf));
+  const float dd = fmaf(c.dx, Xf, fmaf(c.dy, Yf, c.d0)) - (gp < 0.0 ? -hp : hp);
   const float q = c.wn / dd;
   r.w = c.fac * q * q;
-  r.ok = r.su >= g.lo_u && r.su <= g.hi_u;
   return r;
+}
+
+// Sample position of voxel j of the chunk: row index and fraction, and whether
+// it lies in the v window.  (Identical in both directions.)
+__device__ inline void column_row(const ColumnView& cv, int j, int& row, float& fv, bool& okv) {
+  float f = fmaf((float)j, cv.dsv, cv.fvb);
+  okv = f >= cv.flo && f <= cv.fhi;
+  f = fminf(fmaxf(f, cv.flo), cv.fhi);
+  const float fl = floorf(f);
+  fv = f - fl;
+  row = cv.ivb + (int)fl;
+}
+
+__device__ inline bool in_mask(const BpGrid& g, int x, int y, int z) {
+  if (!g.mask) return true;
+  const int dz = z - g.mcz, dy = y - g.mcy, dx = x - g.mcx;
+  return (float)(dz * dz + dy * dy + dx * dx) <= g.mask_r2;
+}
+
+// The v coordinate of voxel z is formed from a base taken at the 16-aligned z
+// at or below it (zb), whatever the thread's z-chunk, so every chunking and
+// tile shape evaluates the same floating-point expression for each voxel and
+// the volume is bitwise independent of the launch tuning.
+constexpr int kZBase = 16;
+
+struct ColumnPos {
+  int x, y, z0;
+  int joff;           // z0 - zb
+  double X, Y, Zb;    // Zb: Z of the base voxel zb
+  float Xf, Yf;
+};
+
+__device__ inline bool column_pos(const BpGrid& g, int tile_x, int tiles_x, int tiles_y, int zc_size,
+                                  ColumnPos& p) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int per_z = tiles_x * tiles_y;
+  const int zc = t / per_z;
+  const int rem = t - zc * per_z;
+  const int ty_t = rem / tiles_x, tx_t = rem - ty_t * tiles_x;
+  const int tile_y = 256 / tile_x;
+  p.x = g.x0 + tx_t * tile_x + (int)(threadIdx.x % tile_x);
+  p.y = ty_t * tile_y + (int)(threadIdx.x / tile_x);
+  p.z0 = zc * zc_size;
+  const int zb = (p.z0 / kZBase) * kZBase;
+  p.joff = p.z0 - zb;
+  // recon/bp3d20_325.cpp:107-109 form the coordinate as start + index * step, unfused
+  p.X = __dadd_rn(g.gx0, __dmul_rn((double)p.x, g.gsx));
+  p.Y = __dadd_rn(g.gy0, __dmul_rn((double)p.y, g.gsy));
+  p.Zb = __dadd_rn(g.gz0, __dmul_rn((double)zb, g.gsz));
+  p.Xf = (float)p.X;
+  p.Yf = (float)p.Y;
+  return p.x < g.x1 && p.y < g.ny;
 }
 
 template <int ZC>
@@ -358,78 +425,61 @@ __global__ __launch_bounds__(256) void backproject_kernel(const float* __restric
                                                           const BpView* __restrict__ views, int view0, int nviews,
                                                           BpGrid g, float* __restrict__ vol, int accumulate,
                                                           int tile_x, int tiles_x, int tiles_y) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int per_z = tiles_x * tiles_y;
-  const int zc = t / per_z;
-  const int rem = t - zc * per_z;
-  const int ty_t = rem / tiles_x, tx_t = rem - ty_t * tiles_x;
-  const int tile_y = 256 / tile_x;
-  const int x = g.x0 + tx_t * tile_x + (int)(threadIdx.x % tile_x);
-  const int y = ty_t * tile_y + (int)(threadIdx.x / tile_x);
-  const int z0 = zc * ZC;
-  if (x >= g.x1 || y >= g.ny) return;
-
-  const float X = fmaf((float)x, g.gsx, g.gx0);
-  const float Y = fmaf((float)y, g.gsy, g.gy0);
-  float Z[ZC];
-  bool inmask[ZC];
+  ColumnPos cp;
+  if (!column_pos(g, tile_x, tiles_x, tiles_y, ZC, cp)) return;
+  bool live[ZC];
 #pragma unroll
-  for (int j = 0; j < ZC; ++j) {
-    Z[j] = fmaf((float)(z0 + j), g.gsz, g.gz0);
-    bool m = (z0 + j) < g.nz;
-    if (g.mask) {
-      const int dz = z0 + j - g.mcz, dy = y - g.mcy, dx = x - g.mcx;
-      m = m && (float)(dz * dz + dy * dy + dx * dx) <= g.mask_r2;
-    }
-    inmask[j] = m;
-  }
+  for (int j = 0; j < ZC; ++j) live[j] = (cp.z0 + j) < g.nz && in_mask(g, cp.x, cp.y, cp.z0 + j);
   float acc[ZC];
 #pragma unroll
   for (int j = 0; j < ZC; ++j) acc[j] = 0.f;
 
   const size_t view_stride = (size_t)g.rows * g.pitch;
   for (int k = view0; k < view0 + nviews; ++k) {
-    const ColumnView cv = column_view(views[k], X, Y, g);
-    const float suc = fminf(fmaxf(cv.su, g.lo_u), g.hi_u);
-    const float fu0 = floorf(suc);
-    const float fu = suc - fu0;
-    const float* base = filt + (size_t)k * view_stride + (int)fu0 + kPadLo;
+    const ColumnView cv = column_view(views[k], cp.X, cp.Y, cp.Xf, cp.Yf, cp.Zb, g);
+    const float* base = filt + (size_t)k * view_stride + (cv.iu + kPadLo);
     const float wu = cv.ok ? cv.w : 0.f;
 #pragma unroll
     for (int j = 0; j < ZC; ++j) {
-      const float sv = fmaf(cv.svk, Z[j], cv.sv0);
-      const bool okv = sv >= g.lo_v && sv <= g.hi_v;
-      const float svc = fminf(fmaxf(sv, g.lo_v), g.hi_v);
-      const float fv0 = floorf(svc);
-      const float fv = svc - fv0;
-      const float* p = base + (size_t)((int)fv0 + kPadLo) * g.pitch;
+      int row;
+      float fv;
+      bool okv;
+      column_row(cv, cp.joff + j, row, fv, okv);
+      const float* p = base + (size_t)(row + kPadLo) * g.pitch;
       const float p00 = p[0], p01 = p[1], p10 = p[g.pitch], p11 = p[g.pitch + 1];
-      const float a = fmaf(fu, p01 - p00, p00);
-      const float b = fmaf(fu, p11 - p10, p10);
+      const float a = fmaf(cv.fu, p01 - p00, p00);
+      const float b = fmaf(cv.fu, p11 - p10, p10);
       const float val = fmaf(fv, b - a, a);
-      acc[j] = fmaf((okv && inmask[j]) ? wu : 0.f, val, acc[j]);
+      acc[j] = fmaf((okv && live[j]) ? wu : 0.f, val, acc[j]);
     }
   }
 #pragma unroll
   for (int j = 0; j < ZC; ++j) {
-    if (z0 + j >= g.nz) break;
-    float* o = vol + ((size_t)(z0 + j) * g.ny + y) * g.nx + x;
+    if (cp.z0 + j >= g.nz) break;
+    float* o = vol + ((size_t)(cp.z0 + j) * g.ny + cp.y) * g.nx + cp.x;
     *o = accumulate ? *o + acc[j] : acc[j];
   }
+}
+
+static void bp_tiling(const BpGrid& grid, const monte_tuning& t, int zc, int& tile_x, int& tiles_x, int& tiles_y,
+                      long& nblocks) {
+  tile_x = t.bp_tile_x > 0 ? t.bp_tile_x : 16;
+  const int tile_y = 256 / tile_x;
+  const int ncols = grid.x1 - grid.x0;
+  tiles_x = (ncols + tile_x - 1) / tile_x;
+  tiles_y = (grid.ny + tile_y - 1) / tile_y;
+  const int zchunks = (grid.nz + zc - 1) / zc;
+  nblocks = (long)tiles_x * tiles_y * zchunks;
 }
 
 hipError_t launch_backproject(const float* filt_pad, const BpView* views, int view0, int nviews_chunk,
                               const BpGrid& grid, float* vol, int accumulate, const monte_tuning& t,
                               hipStream_t s) {
-  const int tile_x = t.bp_tile_x > 0 ? t.bp_tile_x : 16;
-  if (256 % tile_x) return hipErrorInvalidValue;
-  const int tile_y = 256 / tile_x;
   const int zc = t.bp_zchunk > 0 ? t.bp_zchunk : 8;
-  const int ncols = grid.x1 - grid.x0;
-  const int tiles_x = (ncols + tile_x - 1) / tile_x;
-  const int tiles_y = (grid.ny + tile_y - 1) / tile_y;
-  const int zchunks = (grid.nz + zc - 1) / zc;
-  const long nblocks = (long)tiles_x * tiles_y * zchunks;
+  int tile_x, tiles_x, tiles_y;
+  long nblocks;
+  if (t.bp_tile_x > 0 && 256 % t.bp_tile_x) return hipErrorInvalidValue;
+  bp_tiling(grid, t, zc, tile_x, tiles_x, tiles_y, nblocks);
   if (nblocks <= 0) return hipSuccess;
   if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
   dim3 gd((unsigned)nblocks), bd(256);
@@ -443,83 +493,139 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 }
 
 // ------------------------------------------- transposed backprojection
-// The matched projector: every (voxel, view) pair scatters vol * w * bilinear
-// weight onto the same four taps the backprojector gathers from, using the
-// same column_view arithmetic, so <BP q, x> == <q, BP^T x> up to rounding.
-// Taps landing in the guard cells are discarded by the readers.
-template <int ZC>
+// The matched projector.  Every (voxel, view) pair deposits vol * w *
+// bilinear weight on the four taps the backprojector gathers from, with the
+// same column_view / column_row arithmetic, so <BP q, x> == <q, BP^T x> up to
+// the order of the sums.  A block owns a tile of voxel columns x ZT voxels
+// along z (values kept in registers) and walks the views: for each view it
+// finds the tile's detector footprint (block min/max of the taps touched),
+// accumulates into a privatised copy of that footprint in LDS with LDS
+// atomics, then adds the footprint to the global stack with one global atomic
+// per touched pixel.  Footprints larger than the LDS patch fall back to global
+// atomics for that view.  Taps on guard cells are dropped or folded by the
+// callers.
+constexpr int kPatch = 8192;  // LDS floats per block
+
+__device__ inline int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int ZT>
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ vol,
                                                       const BpView* __restrict__ views, int view0, int nviews,
                                                       BpGrid g, float* __restrict__ filt, int tile_x, int tiles_x,
                                                       int tiles_y) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int per_z = tiles_x * tiles_y;
-  const int zc = t / per_z;
-  const int rem = t - zc * per_z;
-  const int ty_t = rem / tiles_x, tx_t = rem - ty_t * tiles_x;
-  const int tile_y = 256 / tile_x;
-  const int x = g.x0 + tx_t * tile_x + (int)(threadIdx.x % tile_x);
-  const int y = ty_t * tile_y + (int)(threadIdx.x / tile_x);
-  const int z0 = zc * ZC;
-  if (x >= g.x1 || y >= g.ny) return;
-
-  const float X = fmaf((float)x, g.gsx, g.gx0);
-  const float Y = fmaf((float)y, g.gsy, g.gy0);
-  float Z[ZC], val[ZC];
+  __shared__ float patch[kPatch];
+  __shared__ int bounds[4];  // umin, umax, vmin, vmax over the block
+  ColumnPos cp;
+  const bool col_ok = column_pos(g, tile_x, tiles_x, tiles_y, ZT, cp);
+  float val[ZT];
+  bool any = false;
 #pragma unroll
-  for (int j = 0; j < ZC; ++j) {
-    Z[j] = fmaf((float)(z0 + j), g.gsz, g.gz0);
-    bool m = (z0 + j) < g.nz;
-    if (g.mask) {
-      const int dz = z0 + j - g.mcz, dy = y - g.mcy, dx = x - g.mcx;
-      m = m && (float)(dz * dz + dy * dy + dx * dx) <= g.mask_r2;
-    }
-    val[j] = m ? vol[((size_t)(z0 + j) * g.ny + y) * g.nx + x] : 0.f;
+  for (int j = 0; j < ZT; ++j) {
+    const int z = cp.z0 + j;
+    val[j] = (col_ok && z < g.nz && in_mask(g, cp.x, cp.y, z)) ? vol[((size_t)z * g.ny + cp.y) * g.nx + cp.x] : 0.f;
+    any = any || val[j] != 0.f;
   }
+  const int tid = threadIdx.x;
   const size_t view_stride = (size_t)g.rows * g.pitch;
   for (int k = view0; k < view0 + nviews; ++k) {
-    const ColumnView cv = column_view(views[k], X, Y, g);
-    if (!cv.ok) continue;
-    const float suc = fminf(fmaxf(cv.su, g.lo_u), g.hi_u);
-    const float fu0 = floorf(suc);
-    const float fu = suc - fu0;
-    float* base = filt + (size_t)k * view_stride + (int)fu0 + kPadLo;
-#pragma unroll
-    for (int j = 0; j < ZC; ++j) {
-      const float sv = fmaf(cv.svk, Z[j], cv.sv0);
-      if (!(sv >= g.lo_v && sv <= g.hi_v) || val[j] == 0.f) continue;
-      const float fv0 = floorf(sv);
-      const float fv = sv - fv0;
-      float* p = base + (size_t)((int)fv0 + kPadLo) * g.pitch;
-      const float s = cv.w * val[j];
-      const float a = s * (1.f - fv), b = s * fv;
-      atomicAdd(p, a * (1.f - fu));
-      atomicAdd(p + 1, a * fu);
-      atomicAdd(p + g.pitch, b * (1.f - fu));
-      atomicAdd(p + g.pitch + 1, b * fu);
+    ColumnView cv{};
+    int umin = 1 << 30, umax = -(1 << 30), vmin = 1 << 30, vmax = -(1 << 30);
+    const bool act = col_ok && any;
+    if (act) {
+      cv = column_view(views[k], cp.X, cp.Y, cp.Xf, cp.Yf, cp.Zb, g);
+      if (cv.ok) {
+        umin = cv.iu;
+        umax = cv.iu + 1;
+        int r0, r1;
+        float f;
+        bool o;
+        column_row(cv, cp.joff, r0, f, o);
+        column_row(cv, cp.joff + ZT - 1, r1, f, o);
+        vmin = min(r0, r1);
+        vmax = max(r0, r1) + 1;
+      }
     }
+    umin = wave_min_i(umin); umax = wave_max_i(umax);
+    vmin = wave_min_i(vmin); vmax = wave_max_i(vmax);
+    if (tid == 0) { bounds[0] = 1 << 30; bounds[1] = -(1 << 30); bounds[2] = 1 << 30; bounds[3] = -(1 << 30); }
+    __syncthreads();
+    if ((tid & 63) == 0) {
+      atomicMin(&bounds[0], umin); atomicMax(&bounds[1], umax);
+      atomicMin(&bounds[2], vmin); atomicMax(&bounds[3], vmax);
+    }
+    __syncthreads();
+    const int u0 = bounds[0], v0 = bounds[2];
+    const int fu = bounds[1] - u0 + 1, fvn = bounds[3] - v0 + 1;
+    if (fu <= 0 || fvn <= 0) {  // nothing of this tile lands on the detector
+      __syncthreads();
+      continue;
+    }
+    float* out = filt + (size_t)k * view_stride;
+    const bool lds = (long)fu * fvn <= kPatch;
+    if (lds)
+      for (int i = tid; i < fu * fvn; i += 256) patch[i] = 0.f;
+    __syncthreads();
+    if (act && cv.ok) {
+#pragma unroll
+      for (int j = 0; j < ZT; ++j) {
+        int row;
+        float fv;
+        bool okv;
+        column_row(cv, cp.joff + j, row, fv, okv);
+        if (!okv || val[j] == 0.f) continue;
+        const float sw = cv.w * val[j];
+        const float b = sw * fv, a = sw - b;
+        const float a1 = a * cv.fu, b1 = b * cv.fu;
+        if (lds) {
+          float* p = patch + (row - v0) * fu + (cv.iu - u0);
+          atomicAdd(p, a - a1);
+          atomicAdd(p + 1, a1);
+          atomicAdd(p + fu, b - b1);
+          atomicAdd(p + fu + 1, b1);
+        } else {
+          float* p = out + (size_t)(row + kPadLo) * g.pitch + cv.iu + kPadLo;
+          atomicAdd(p, a - a1);
+          atomicAdd(p + 1, a1);
+          atomicAdd(p + g.pitch, b - b1);
+          atomicAdd(p + g.pitch + 1, b1);
+        }
+      }
+    }
+    __syncthreads();
+    if (lds) {
+      for (int i = tid; i < fu * fvn; i += 256) {
+        const float v = patch[i];
+        if (v != 0.f) {
+          const int r = i / fu, c = i - r * fu;
+          atomicAdd(out + (size_t)(v0 + r + kPadLo) * g.pitch + u0 + c + kPadLo, v);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
 hipError_t launch_project(const float* vol, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
                           float* filt_pad, const monte_tuning& t, hipStream_t s) {
-  const int tile_x = t.bp_tile_x > 0 ? t.bp_tile_x : 16;
-  if (256 % tile_x) return hipErrorInvalidValue;
-  const int tile_y = 256 / tile_x;
-  const int zc = t.bp_zchunk > 0 ? t.bp_zchunk : 8;
-  const int ncols = grid.x1 - grid.x0;
-  const int tiles_x = (ncols + tile_x - 1) / tile_x;
-  const int tiles_y = (grid.ny + tile_y - 1) / tile_y;
-  const int zchunks = (grid.nz + zc - 1) / zc;
-  const long nblocks = (long)tiles_x * tiles_y * zchunks;
+  constexpr int ZT = 16;
+  int tile_x, tiles_x, tiles_y;
+  long nblocks;
+  monte_tuning tt = t;
+  tt.bp_tile_x = 16;
+  bp_tiling(grid, tt, ZT, tile_x, tiles_x, tiles_y, nblocks);
   if (nblocks <= 0) return hipSuccess;
-  dim3 gd((unsigned)nblocks), bd(256);
-  switch (zc) {
-    case 4: hipLaunchKernelGGL((project_kernel<4>), gd, bd, 0, s, vol, views, view0, nviews_chunk, grid, filt_pad, tile_x, tiles_x, tiles_y); break;
-    case 8: hipLaunchKernelGGL((project_kernel<8>), gd, bd, 0, s, vol, views, view0, nviews_chunk, grid, filt_pad, tile_x, tiles_x, tiles_y); break;
-    case 16: hipLaunchKernelGGL((project_kernel<16>), gd, bd, 0, s, vol, views, view0, nviews_chunk, grid, filt_pad, tile_x, tiles_x, tiles_y); break;
-    default: return hipErrorInvalidValue;
-  }
+  if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((project_kernel<ZT>), dim3((unsigned)nblocks), dim3(256), 0, s, vol, views, view0,
+                     nviews_chunk, grid, filt_pad, tile_x, tiles_x, tiles_y);
   return hipGetLastError();
 }
 

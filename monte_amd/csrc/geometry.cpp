@@ -24,45 +24,49 @@ void compute_bp_views(const monte_geom& g, std::vector<BpView>& out) {
     BpView& v = out[k];
     // U = X cos b + Y sin b + sod: distance from the source plane
     // (recon/bp3d20_325.cpp:111-118 rotates P - S by -beta and divides by it).
-    v.cx = (float)cb;
-    v.cy = (float)sb;
-    v.c0 = (float)g.sod;
+    v.cx = cb;
+    v.cy = sb;
+    v.c0 = g.sod;
     // in-plane numerator y' = -X sin b + Y cos b
-    v.ax = (float)(-sb);
-    v.ay = (float)cb;
+    v.ax = -sb;
+    v.ay = cb;
     if (g.preset == MONTE_PRESET_BP3D20) {
       // recon/bp3d20_325.cpp:134-135: y = -(w_y/pitch - H/2), w_y = y' sdd / U
-      v.su0 = (float)(0.5 * g.nu);
-      v.suk = (float)(-g.sdd / g.du);
-      v.sv0 = (float)(0.5 * g.nv);
-      v.svk = (float)(-g.sdd / g.dv);
+      v.su0 = 0.5 * g.nu;
+      v.suk = -g.sdd / g.du;
+      v.sv0 = 0.5 * g.nv;
+      v.svk = -g.sdd / g.dv;
       // :145-153: d = sgn(s') |s' - T t'| / sqrt(1 + T^2), s' = X cos - Y sin,
       // t' = X sin + Y cos, T = tan(beta) with beta the integer degree count
-      // taken as radians; weight 60^2 / (60 - d)^2 (:169).
-      const double T = std::tan((double)(k * g.step_deg));
+      // taken as radians; weight 60^2 / (60 - d)^2 (:169).  cos and sin here
+      // are computed as the reference computes them, cos(M_PI*beta/180).
+      const int beta = k * g.step_deg;
+      const double cr = std::cos(kPi * beta / 180), sr = std::sin(kPi * beta / 180);
+      const double T = std::tan((double)beta);
       const double kd = 1.0 / std::sqrt(1.0 + T * T);
       v.wn = (float)g.odd;
       v.d0 = (float)g.odd;
       v.dx = 0.f;
       v.dy = 0.f;
-      v.gx = (float)cb;
-      v.gy = (float)(-sb);
-      v.hx = (float)(kd * (cb - T * sb));
-      v.hy = (float)(kd * (-sb - T * cb));
+      v.gx = cr;
+      v.gy = -sr;
+      v.hx = (float)(kd * (cr - T * sr));
+      v.hy = (float)(kd * (-sr - T * cr));
       // :169-170: * beta_span * 2 pi / 360 * 2.7 * 5
       v.fac = (float)(g.step_deg * 2.0 * kPi / 360.0 * g.scale);
     } else {
       // pixel centres: sample index = u_idx - 1/2
-      v.su0 = (float)(0.5 * g.nu - g.u_off / g.du - 0.5);
-      v.suk = (float)(g.sdd / g.du);
-      v.sv0 = (float)(0.5 * g.nv - g.v_off / g.dv - 0.5);
-      v.svk = (float)(g.sdd / g.dv);
+      v.su0 = 0.5 * g.nu - g.u_off / g.du - 0.5;
+      v.suk = g.sdd / g.du;
+      v.sv0 = 0.5 * g.nv - g.v_off / g.dv - 0.5;
+      v.svk = g.sdd / g.dv;
       // FDK distance weight (sod / U)^2 and angular step
       v.wn = (float)g.sod;
       v.d0 = (float)g.sod;
       v.dx = (float)cb;
       v.dy = (float)sb;
-      v.gx = v.gy = v.hx = v.hy = 0.f;
+      v.gx = v.gy = 0.0;
+      v.hx = v.hy = 0.f;
       v.fac = (float)g.dangle;
     }
   }
@@ -75,23 +79,23 @@ BpGrid compute_bp_grid(const monte_geom& g) {
   q.rows = g.nv + kPadLo + kPadHi;
   if (g.preset == MONTE_PRESET_BP3D20) {
     // recon/bp3d20_325.cpp:107-109: X = -12.8 + 0.1 s, Y = 12.8 - 0.1 t, Z = 12.8 - 0.1 z
-    q.gx0 = (float)(g.ox - 0.5 * g.nx * g.vx); q.gsx = (float)g.vx;
-    q.gy0 = (float)(g.oy + 0.5 * g.ny * g.vy); q.gsy = (float)(-g.vy);
-    q.gz0 = (float)(g.oz + 0.5 * g.nz * g.vz); q.gsz = (float)(-g.vz);
+    q.gx0 = g.ox - 0.5 * g.nx * g.vx; q.gsx = g.vx;
+    q.gy0 = g.oy + 0.5 * g.ny * g.vy; q.gsy = -g.vy;
+    q.gz0 = g.oz + 0.5 * g.nz * g.vz; q.gsz = -g.vz;
     // :127 and :162: |w| <= H/2 pitch and 0 <= x <= H, inclusive
-    q.lo_u = 0.f; q.hi_u = (float)g.nu;
-    q.lo_v = 0.f; q.hi_v = (float)g.nv;
+    q.lo_u = 0.0; q.hi_u = g.nu;
+    q.lo_v = 0.0; q.hi_v = g.nv;
     q.x0 = g.roi_x0; q.x1 = g.roi_x1;
     q.mask = g.mask_r >= 0 ? 1 : 0;
     q.mask_r2 = (float)(g.mask_r * g.mask_r);
     q.mcx = g.nx / 2; q.mcy = g.ny / 2; q.mcz = g.nz / 2;
   } else {
-    q.gx0 = (float)(g.ox - 0.5 * (g.nx - 1) * g.vx); q.gsx = (float)g.vx;
-    q.gy0 = (float)(g.oy - 0.5 * (g.ny - 1) * g.vy); q.gsy = (float)g.vy;
-    q.gz0 = (float)(g.oz - 0.5 * (g.nz - 1) * g.vz); q.gsz = (float)g.vz;
+    q.gx0 = g.ox - 0.5 * (g.nx - 1) * g.vx; q.gsx = g.vx;
+    q.gy0 = g.oy - 0.5 * (g.ny - 1) * g.vy; q.gsy = g.vy;
+    q.gz0 = g.oz - 0.5 * (g.nz - 1) * g.vz; q.gsz = g.vz;
     // support of the zero-padded bilinear interpolant
-    q.lo_u = -1.f; q.hi_u = (float)g.nu;
-    q.lo_v = -1.f; q.hi_v = (float)g.nv;
+    q.lo_u = -1.0; q.hi_u = g.nu;
+    q.lo_v = -1.0; q.hi_v = g.nv;
     q.x0 = 0; q.x1 = g.nx;
     q.mask = 0;
   }

@@ -27,28 +27,34 @@ constexpr int kPadHi = 3;  // and after (covers the +1 bilinear tap at the windo
 //   w  = fac (wn / Dd)^2                       distance weight x angular step
 // Standard preset: Dd = U, wn = sod, fac = dangle.  bp3d20 preset: Dd = 60 - d
 // with d the signed point-line distance of recon/bp3d20_325.cpp:145-153.
+// The sample coordinates are formed per voxel column in FP64 (U, 1/U, su and
+// the first sv of the column's z-chunk), then split into an integer pixel
+// index and an FP32 fraction, so the per-voxel FP32 work only moves a small
+// fraction along v.  The bp3d20 sign test sgn(gx X + gy Y) is evaluated in
+// unfused FP64 exactly as the reference evaluates it: at columns where it is
+// zero in exact arithmetic (e.g. X = -Y at 135 degrees) the reference's
+// rounding decides the sign of d, and with it the weight.
 struct BpView {
-  float cx, cy, c0;
-  float ax, ay;
-  float su0, suk;
-  float sv0, svk;
+  double cx, cy, c0;
+  double ax, ay;
+  double su0, suk;
+  double sv0, svk;
+  double gx, gy;
   float wn, d0, dx, dy;
-  float gx, gy, hx, hy;
-  float fac;
-  float pad0, pad1;
+  float hx, hy, fac, pad0;
 };
-static_assert(sizeof(BpView) == 80, "BpView layout");
+static_assert(sizeof(BpView) == 120, "BpView layout");
 
 // Geometry-wide constants of the backprojector.
 struct BpGrid {
   int nx, ny, nz, nu, nv;
   int pitch;          // padded row pitch of filtered projections (nu + kPadLo + kPadHi)
   int rows;           // padded rows (nv + kPadLo + kPadHi)
-  float gx0, gsx;     // X = gx0 + x * gsx
-  float gy0, gsy;
-  float gz0, gsz;
-  float lo_u, hi_u;   // valid sample window along u (inclusive)
-  float lo_v, hi_v;
+  double gx0, gsx;    // X = gx0 + x * gsx (unfused, as recon/bp3d20_325.cpp:107)
+  double gy0, gsy;
+  double gz0, gsz;
+  double lo_u, hi_u;  // valid sample window along u (inclusive)
+  double lo_v, hi_v;
   int x0, x1;         // x range reconstructed (bp3d20 ROI)
   int mask;           // 1: sphere mask in voxel index space
   float mask_r2;
@@ -102,6 +108,14 @@ struct McJob {
   uint32_t batches_per_view;     // ceil(photons_per_view / batch)
   uint64_t unit_begin;           // first global work unit of this job
   uint64_t unit_count;
+};
+
+// FP64 ray geometry of the exact primary estimator.
+struct PrimaryGeom {
+  double sod, sdd, un0, u_step, vn0, v_step;
+  double b0[3], b1[3], vox[3];
+  int n[3];
+  int nu, nv, n_views;
 };
 
 struct McCounters {  // device-side statistics, one per job
@@ -180,7 +194,9 @@ hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* 
 hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, unsigned int* queue,
                                McCounters* counters, int batch, int block, int grid, hipStream_t s);
 hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs, hipStream_t s);
-hipError_t launch_primary_exact(const McJob& job, const float* spec_w, int n_spec, float* out, hipStream_t s);
+hipError_t launch_primary_exact(const PrimaryGeom& P, const double* cs64, const uint8_t* labels,
+                                const McTables* tables, float e_kev, const float* spec_w, int use_spec, float* out,
+                                hipStream_t s);
 hipError_t launch_counts_to_lineint(const double* counts, float* out, int n_views, int nu, int nv,
                                     uint64_t photons_per_view, hipStream_t s);
 

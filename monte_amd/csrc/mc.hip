@@ -10,14 +10,17 @@
 //     history can be replayed and results do not depend on scheduling;
 //   * the kernel is persistent: waves pull batches of photons (work units)
 //     from a device queue shared by all jobs of a launch;
-//   * the loop advances every live lane by one Woodcock step per iteration;
-//     lanes whose photon has been absorbed, capped or tallied are refilled
-//     at once from the wave's current batch (ballot + mbcnt prefix ranks), so
-//     the wave stays full until the queue drains;
+//   * each loop iteration advances every flying lane by one Woodcock step.
+//     Lanes that collide (real interaction pending) or die (absorbed, capped,
+//     tallied) park; the wave runs the interaction code, or refills dead lanes
+//     from its batch (ballot + mbcnt prefix ranks), only once enough lanes
+//     are parked or nothing else can run, so those branches execute near full
+//     width instead of for one lane at a time;
 //   * flights are clipped to the phantom box (no Woodcock steps through air,
 //     quirk Q9) and the first step starts at the box entry;
 //   * primary and scatter counts are tallied with FP64 atomics.  Every tally
-//     adds 1.0, so sums are exact integers and independent of order.
+//     adds 1.0, so sums are exact integers and independent of order;
+//   * statistics are counted per wave from ballots in scalar registers.
 #include "internal.h"
 
 namespace monte {
@@ -37,15 +40,17 @@ __device__ inline void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32
   }
 }
 
+// A photon's stream: counter (ctr, photon, view, job); the four outputs of the
+// last block are consumed in order.
 struct Stream {
-  uint32_t ctr, photon, view, job;  // counter words
-  uint32_t b0, b1, b2, b3;          // unused outputs of the last block
+  uint32_t ctr, photon, view;
+  uint32_t b0, b1, b2, b3;
   int left;
 };
 
-__device__ inline float next_u01(Stream& s, uint32_t k0, uint32_t k1) {
+__device__ inline float next_u01(Stream& s, uint32_t job, uint32_t k0, uint32_t k1) {
   if (s.left == 0) {
-    uint32_t c0 = s.ctr, c1 = s.photon, c2 = s.view, c3 = s.job;
+    uint32_t c0 = s.ctr, c1 = s.photon, c2 = s.view, c3 = job;
     philox10(c0, c1, c2, c3, k0, k1);
     s.b0 = c0; s.b1 = c1; s.b2 = c2; s.b3 = c3;
     s.ctr += 1;
@@ -60,20 +65,20 @@ __device__ inline float next_u01(Stream& s, uint32_t k0, uint32_t k1) {
 // --------------------------------------------------------------- physics
 // Kahn's rejection sampling of the Klein-Nishina distribution as written in
 // CBCT_real325.cu:687-722; returns rho = lambda'/lambda.
-__device__ inline float kahn(float e_kev, Stream& s, uint32_t k0, uint32_t k1) {
+__device__ inline float kahn(float e_kev, Stream& s, uint32_t job, uint32_t k0, uint32_t k1) {
   const float lambda = 511.0f / e_kev;
   const float p1 = (lambda + 2.0f) / (9.0f * lambda + 2.0f);
   for (int guard = 0; guard < 1000; ++guard) {
-    const float r1 = next_u01(s, k0, k1);
+    const float r1 = next_u01(s, job, k0, k1);
     if (r1 < p1) {
-      const float r2 = next_u01(s, k0, k1);
+      const float r2 = next_u01(s, job, k0, k1);
       const float ro = 1.0f + (2.0f / lambda) * r2;
-      const float r3 = next_u01(s, k0, k1);
+      const float r3 = next_u01(s, job, k0, k1);
       if (r3 <= 4.0f * ((1.0f / ro) - (1.0f / (ro * ro)))) return ro;
     } else {
-      const float r2 = next_u01(s, k0, k1);
+      const float r2 = next_u01(s, job, k0, k1);
       const float ro = (lambda + 2.0f) / (lambda + 2.0f * (1.0f - r2));
-      const float r3 = next_u01(s, k0, k1);
+      const float r3 = next_u01(s, job, k0, k1);
       const float q = lambda - ro * lambda + 1.0f;
       if (r3 <= 0.5f * (q * q + (1.0f / ro))) return ro;
     }
@@ -115,52 +120,27 @@ __device__ inline int ebin_of(float e) {
   return b < 1 ? 1 : (b > kRows - 1 ? kRows - 1 : b);
 }
 
-// Slab test: parametric interval of p + t d inside the box.
+// Slab test along one axis, folded into [lo, hi]; false if the ray misses.
+__device__ inline bool slab(float p, float d, float b0, float b1, float& lo, float& hi) {
+  if (d != 0.f) {
+    const float inv = 1.0f / d;
+    float ta = (b0 - p) * inv, tb = (b1 - p) * inv;
+    lo = fmaxf(lo, fminf(ta, tb));
+    hi = fminf(hi, fmaxf(ta, tb));
+    return true;
+  }
+  return p >= b0 && p < b1;
+}
+
+// Parametric interval of p + t d inside the phantom box.
 __device__ inline bool ray_box(float px, float py, float pz, float dx, float dy, float dz, const McJob& J,
                                float& t0, float& t1) {
   float lo = -INFINITY, hi = INFINITY;
-  const float p[3] = {px, py, pz}, d[3] = {dx, dy, dz};
-  const float b0[3] = {J.bx0, J.by0, J.bz0}, b1[3] = {J.bx1, J.by1, J.bz1};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (d[a] != 0.f) {
-      const float inv = 1.0f / d[a];
-      float ta = (b0[a] - p[a]) * inv, tb = (b1[a] - p[a]) * inv;
-      if (ta > tb) { const float tt = ta; ta = tb; tb = tt; }
-      lo = fmaxf(lo, ta);
-      hi = fminf(hi, tb);
-    } else if (p[a] < b0[a] || p[a] >= b1[a]) {
-      return false;
-    }
-  }
-  if (!(hi > lo)) return false;
+  const bool ok = slab(px, dx, J.bx0, J.bx1, lo, hi) && slab(py, dy, J.by0, J.by1, lo, hi) &&
+                  slab(pz, dz, J.bz0, J.bz1, lo, hi);
+  if (!ok || !(hi > lo)) return false;
   t0 = lo; t1 = hi;
   return true;
-}
-
-struct LaneCounters {
-  uint32_t histories, steps, interactions, photo, coherent, compton, tp, ts, missed, capped;
-};
-
-__device__ inline unsigned long long wave_sum(uint32_t v) {
-  unsigned long long x = v;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-
-__device__ inline void flush_counters(LaneCounters& c, McCounters* dst) {
-  const unsigned long long h = wave_sum(c.histories), st = wave_sum(c.steps), in = wave_sum(c.interactions),
-                           ph = wave_sum(c.photo), co = wave_sum(c.coherent), cp = wave_sum(c.compton),
-                           tp = wave_sum(c.tp), ts = wave_sum(c.ts), mi = wave_sum(c.missed),
-                           ca = wave_sum(c.capped);
-  if (__lane_id() == 0) {
-    atomicAdd(&dst->histories, h); atomicAdd(&dst->steps, st); atomicAdd(&dst->interactions, in);
-    atomicAdd(&dst->photo, ph); atomicAdd(&dst->coherent, co); atomicAdd(&dst->compton, cp);
-    atomicAdd(&dst->tallied_primary, tp); atomicAdd(&dst->tallied_scatter, ts);
-    atomicAdd(&dst->missed, mi); atomicAdd(&dst->capped, ca);
-  }
-  c = LaneCounters{};
 }
 
 __device__ inline int job_of_unit(const McJob* __restrict__ jobs, int n_jobs, uint64_t u) {
@@ -178,33 +158,37 @@ struct Photon {
   float dx, dy, dz;    // unit direction (world)
   float e;             // energy (keV)
   float t, t1;         // flight parameter and box-exit parameter
-  float mumax, inv_mumax;
-  float cb, sb;        // view rotation
+  float mumax;         // Woodcock majorant of the current table row
+  float inv_mumax;
   int eb;              // table row
+  int lab;             // label at the pending collision
   int nint;            // real interactions so far
   int view;            // tally view index (absolute)
 };
 
+enum { kDead = 0, kFly = 1, kCollide = 2 };
+enum { kTallyMiss = 0, kTallyPrimary = 1, kTallyScatter = 2 };
+
 // Straight line from the current position to the detector plane x' = sdd - sod.
-__device__ inline void tally(const Photon& p, const McJob& J, LaneCounters& cnt) {
-  const float xq = p.px * p.cb + p.py * p.sb, yq = -p.px * p.sb + p.py * p.cb;
-  const float ddx = p.dx * p.cb + p.dy * p.sb, ddy = -p.dx * p.sb + p.dy * p.cb;
-  if (!(ddx > 0.f)) { cnt.missed++; return; }
+__device__ inline int tally(const Photon& p, const McJob& J) {
+  const float2 cs = J.view_cs[p.view];
+  const float xq = p.px * cs.x + p.py * cs.y, yq = -p.px * cs.y + p.py * cs.x;
+  const float ddx = p.dx * cs.x + p.dy * cs.y, ddy = -p.dx * cs.y + p.dy * cs.x;
+  if (!(ddx > 0.f)) return kTallyMiss;
   const float tt = ((J.sdd - J.sod) - xq) / ddx;
   const float yh = fmaf(tt, ddy, yq), zh = fmaf(tt, p.dz, p.pz);
   const float ui = fmaf(J.us, yh, J.un0), vi = fmaf(J.vs, zh, J.vn0);
-  if (!(ui >= 0.f && ui < (float)J.nu && vi >= 0.f && vi < (float)J.nv)) { cnt.missed++; return; }
+  if (!(ui >= 0.f && ui < (float)J.nu && vi >= 0.f && vi < (float)J.nv)) return kTallyMiss;
   int iu = (int)floorf(ui), iv = (int)floorf(vi);
   iu = iu > J.nu - 1 ? J.nu - 1 : iu;
   iv = iv > J.nv - 1 ? J.nv - 1 : iv;
   const size_t idx = ((size_t)p.view * J.nu + iu) * J.nv + iv;
   if (p.nint == 0) {
     unsafeAtomicAdd(J.primary + idx, 1.0);
-    cnt.tp++;
-  } else {
-    unsafeAtomicAdd(J.scatter + idx, 1.0);
-    cnt.ts++;
+    return kTallyPrimary;
   }
+  unsafeAtomicAdd(J.scatter + idx, 1.0);
+  return kTallyScatter;
 }
 
 // Starts a flight from the current position; returns false if the photon
@@ -225,25 +209,58 @@ __device__ inline bool start_flight(Photon& p, const McJob& J) {
   return true;
 }
 
+// Wave-uniform statistics (scalar registers), flushed per job.
+struct WaveCounters {
+  uint32_t histories, steps, interactions, photo, coherent, compton, tp, ts, missed, capped;
+};
+
+__device__ inline unsigned popc(uint64_t m) { return (unsigned)__popcll(m); }
+
+__device__ inline void count_tally(WaveCounters& c, bool did, int code) {
+  c.tp += popc(__ballot(did && code == kTallyPrimary));
+  c.ts += popc(__ballot(did && code == kTallyScatter));
+  c.missed += popc(__ballot(did && code == kTallyMiss));
+}
+
+__device__ inline void flush_counters(WaveCounters& c, McCounters* dst) {
+  if (__lane_id() == 0) {
+    using u64 = unsigned long long;
+    atomicAdd(&dst->histories, (u64)c.histories); atomicAdd(&dst->steps, (u64)c.steps);
+    atomicAdd(&dst->interactions, (u64)c.interactions); atomicAdd(&dst->photo, (u64)c.photo);
+    atomicAdd(&dst->coherent, (u64)c.coherent); atomicAdd(&dst->compton, (u64)c.compton);
+    atomicAdd(&dst->tallied_primary, (u64)c.tp); atomicAdd(&dst->tallied_scatter, (u64)c.ts);
+    atomicAdd(&dst->missed, (u64)c.missed); atomicAdd(&dst->capped, (u64)c.capped);
+  }
+  c = WaveCounters{};
+}
+
+// A parked branch runs once at least this many lanes wait for it (or nothing
+// else can run).
+constexpr unsigned kInteractBatch = 16;
+constexpr unsigned kRefillBatch = 16;
+
 __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restrict__ jobs, int n_jobs,
                                                            uint64_t total_units, unsigned int* __restrict__ queue,
                                                            McCounters* __restrict__ counters, int batch) {
-  const uint64_t full = ~0ull;
   int job = -1;
   uint32_t next = 0, end = 0;  // photon range of the current unit (wave-uniform)
   int unit_view = 0;
   long long pending = -1;
   bool exhausted = false;
 
-  bool alive = false;
+  int state = kDead;
   Photon p{};
   Stream rs{};
-  LaneCounters cnt{};
-  McJob J{};
+  WaveCounters cnt{};
+  const McJob* Jp = jobs;
   const int lane = __lane_id();
 
   for (;;) {
-    const uint64_t dead = __ballot(!alive);
+    uint64_t dead = __ballot(state == kDead);
+    const uint64_t fly = __ballot(state == kFly);
+    const uint64_t coll = __ballot(state == kCollide);
+    const bool all_dead = (fly | coll) == 0;
+
     // 1) secure a work unit when lanes are idle and the current one is spent
     if (dead != 0 && next >= end) {
       if (pending < 0 && !exhausted) {
@@ -254,131 +271,153 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       }
       if (pending >= 0) {
         const int pj = __builtin_amdgcn_readfirstlane(job_of_unit(jobs, n_jobs, (uint64_t)pending));
-        if (pj == job || dead == full) {
+        if (pj == job || all_dead) {
           if (pj != job) {
             if (job >= 0) flush_counters(cnt, counters + job);
             job = pj;
-            J = jobs[job];
+            Jp = jobs + job;
           }
-          const uint64_t local = (uint64_t)pending - J.unit_begin;
-          const uint32_t vl = (uint32_t)(local / J.batches_per_view);
-          const uint32_t bi = (uint32_t)(local - (uint64_t)vl * J.batches_per_view);
-          unit_view = J.view_begin + (int)vl;
+          const uint64_t local = (uint64_t)pending - Jp->unit_begin;
+          const uint32_t vl = (uint32_t)(local / Jp->batches_per_view);
+          const uint32_t bi = (uint32_t)(local - (uint64_t)vl * Jp->batches_per_view);
+          unit_view = Jp->view_begin + (int)vl;
           next = bi * (uint32_t)batch;
-          end = min(next + (uint32_t)batch, J.photons_per_view);
+          end = min(next + (uint32_t)batch, Jp->photons_per_view);
           pending = -1;
         }
       }
     }
-    if (dead == full && next >= end && pending < 0 && exhausted) break;
+    if (all_dead && next >= end && pending < 0 && exhausted) break;
 
     // 2) refill dead lanes from the current unit
-    if (dead != 0 && next < end) {
+    if (dead != 0 && next < end && (popc(dead) >= kRefillBatch || fly == 0)) {
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
-      const uint32_t ndead = (uint32_t)__popcll(dead);
-      if (!alive) {
+      bool born = false, tallied = false;
+      int code = 0;
+      if (state == kDead) {
         const uint32_t id = next + rank;
         if (id < end) {
-          cnt.histories++;
-          rs.ctr = 0; rs.photon = id; rs.view = (uint32_t)unit_view; rs.job = J.job_id; rs.left = 0;
+          born = true;
+          rs.ctr = 0; rs.photon = id; rs.view = (uint32_t)unit_view; rs.left = 0;
           // aim at the centre of pixel id mod (nu*nv) (CBCT_real325.cu:464-466)
-          const uint32_t npix = (uint32_t)(J.nu * J.nv);
+          const uint32_t npix = (uint32_t)(Jp->nu * Jp->nv);
           const uint32_t pix = id % npix;
-          const int iu = (int)(pix / (uint32_t)J.nv), iv = (int)(pix - (uint32_t)iu * (uint32_t)J.nv);
-          const float yp = ((float)iu + 0.5f - J.un0) * J.u_step;
-          const float zp = ((float)iv + 0.5f - J.vn0) * J.v_step;
-          const float in = rsqrtf(J.sdd * J.sdd + yp * yp + zp * zp);
-          const float2 cs = J.view_cs[unit_view];
-          p.cb = cs.x; p.sb = cs.y;
+          const int iu = (int)(pix / (uint32_t)Jp->nv), iv = (int)(pix - (uint32_t)iu * (uint32_t)Jp->nv);
+          const float yp = ((float)iu + 0.5f - Jp->un0) * Jp->u_step;
+          const float zp = ((float)iv + 0.5f - Jp->vn0) * Jp->v_step;
+          const float in = rsqrtf(Jp->sdd * Jp->sdd + yp * yp + zp * zp);
+          const float2 cs = Jp->view_cs[unit_view];
           p.view = unit_view;
-          p.dx = (J.sdd * cs.x - yp * cs.y) * in;
-          p.dy = (J.sdd * cs.y + yp * cs.x) * in;
+          p.dx = (Jp->sdd * cs.x - yp * cs.y) * in;
+          p.dy = (Jp->sdd * cs.y + yp * cs.x) * in;
           p.dz = zp * in;
-          p.px = -J.sod * cs.x; p.py = -J.sod * cs.y; p.pz = 0.f;
-          float e = J.e_kev;
-          if (J.spectrum_cdf) {
-            const float u = next_u01(rs, J.key0, J.key1);
+          p.px = -Jp->sod * cs.x; p.py = -Jp->sod * cs.y; p.pz = 0.f;
+          float e = Jp->e_kev;
+          if (Jp->spectrum_cdf) {
+            const float u = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
             int lo = 1, hi = kRows - 1;
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
-              if (J.spectrum_cdf[mid] < u) lo = mid + 1; else hi = mid;
+              if (Jp->spectrum_cdf[mid] < u) lo = mid + 1; else hi = mid;
             }
             e = (float)lo;
           }
           p.e = e;
           p.eb = ebin_of(e);
           p.nint = 0;
-          if (start_flight(p, J)) {
-            alive = true;
+          if (start_flight(p, *Jp)) {
+            state = kFly;
           } else {
-            tally(p, J, cnt);
+            code = tally(p, *Jp);
+            tallied = true;
           }
         }
       }
-      next = min(end, next + ndead);
+      cnt.histories += popc(__ballot(born));
+      count_tally(cnt, tallied, code);
+      next = min(end, next + popc(dead));
     }
 
-    // 3) one Woodcock step for every live lane
-    if (alive) {
-      const float u = next_u01(rs, J.key0, J.key1);
-      p.t = fmaf(-__logf(u), p.inv_mumax, p.t);
-      cnt.steps++;
-      if (p.t >= p.t1) {
-        tally(p, J, cnt);
-        alive = false;
-      } else {
-        const float qx = fmaf(p.t, p.dx, p.px), qy = fmaf(p.t, p.dy, p.py), qz = fmaf(p.t, p.dz, p.pz);
-        int ix = (int)floorf((qx - J.bx0) * J.inv_vx);
-        int iy = (int)floorf((qy - J.by0) * J.inv_vy);
-        int iz = (int)floorf((qz - J.bz0) * J.inv_vz);
-        ix = min(max(ix, 0), J.nx - 1);
-        iy = min(max(iy, 0), J.ny - 1);
-        iz = min(max(iz, 0), J.nz - 1);
-        const int lab = J.labels[((size_t)iz * J.ny + iy) * J.nx + ix];
-        const float4 m = J.tables->m[lab][p.eb];
-        const float u2 = next_u01(rs, J.key0, J.key1);
-        if (u2 * p.mumax < m.x) {
-          // real collision
-          p.px = qx; p.py = qy; p.pz = qz;
-          if (p.nint >= J.max_int) {
-            cnt.capped++;
-            alive = false;
-          } else {
-            const float u3 = next_u01(rs, J.key0, J.key1);
-            cnt.interactions++;
-            if (u3 <= m.y) {
-              cnt.photo++;
-              alive = false;
-            } else {
-              if (u3 <= m.z) {
-                cnt.coherent++;
-                if (J.coherent) {
-                  const float cth = thomson_mu(next_u01(rs, J.key0, J.key1));
-                  const float phi = 6.283185307179586f * next_u01(rs, J.key0, J.key1);
-                  rotate_dir(p.dx, p.dy, p.dz, cth, phi);
-                }
-              } else {
-                cnt.compton++;
-                const float ro = kahn(p.e, rs, J.key0, J.key1);
-                const float lam = 511.0f / p.e, lamd = ro * lam;
-                float cth = 1.0f - (lamd - lam);
-                cth = cth < -1.f ? -1.f : cth;
-                p.e = 511.0f / lamd;
-                p.eb = ebin_of(p.e);
-                const float phi = 6.283185307179586f * next_u01(rs, J.key0, J.key1);
-                rotate_dir(p.dx, p.dy, p.dz, cth, phi);
-              }
-              p.nint += 1;
-              if (!start_flight(p, J)) {
-                tally(p, J, cnt);
-                alive = false;
-              }
+    // 3) interactions of the parked colliding lanes
+    if (coll != 0 && (popc(coll) >= kInteractBatch || fly == 0)) {
+      bool is_photo = false, is_coh = false, is_comp = false, capped = false, tallied = false;
+      int code = 0;
+      if (state == kCollide && p.nint >= Jp->max_int) {
+        capped = true;
+        state = kDead;
+      }
+      if (state == kCollide) {
+        const float4 m = Jp->tables->m[p.lab][p.eb];
+        const float u3 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+        if (u3 <= m.y) {
+          is_photo = true;
+          state = kDead;
+        } else {
+          if (u3 <= m.z) {
+            is_coh = true;
+            if (Jp->coherent) {
+              const float cth = thomson_mu(next_u01(rs, Jp->job_id, Jp->key0, Jp->key1));
+              const float phi = 6.283185307179586f * next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+              rotate_dir(p.dx, p.dy, p.dz, cth, phi);
             }
+          } else {
+            is_comp = true;
+            const float ro = kahn(p.e, rs, Jp->job_id, Jp->key0, Jp->key1);
+            const float lam = 511.0f / p.e, lamd = ro * lam;
+            float cth = 1.0f - (lamd - lam);
+            cth = cth < -1.f ? -1.f : cth;
+            p.e = 511.0f / lamd;
+            p.eb = ebin_of(p.e);
+            const float phi = 6.283185307179586f * next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+            rotate_dir(p.dx, p.dy, p.dz, cth, phi);
+          }
+          p.nint += 1;
+          if (start_flight(p, *Jp)) {
+            state = kFly;
+          } else {
+            code = tally(p, *Jp);
+            tallied = true;
+            state = kDead;
           }
         }
       }
+      cnt.capped += popc(__ballot(capped));
+      cnt.photo += popc(__ballot(is_photo));
+      cnt.coherent += popc(__ballot(is_coh));
+      cnt.compton += popc(__ballot(is_comp));
+      cnt.interactions += popc(__ballot(is_photo || is_coh || is_comp));
+      count_tally(cnt, tallied, code);
     }
+
+    // 4) one Woodcock step for every flying lane
+    bool stepped = false, tallied = false;
+    int code = 0;
+    if (state == kFly) {
+      stepped = true;
+      const float u = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+      p.t = fmaf(-__logf(u), p.inv_mumax, p.t);
+      if (p.t >= p.t1) {
+        code = tally(p, *Jp);
+        tallied = true;
+        state = kDead;
+      } else {
+        const float qx = fmaf(p.t, p.dx, p.px), qy = fmaf(p.t, p.dy, p.py), qz = fmaf(p.t, p.dz, p.pz);
+        const int ix = min(max((int)floorf((qx - Jp->bx0) * Jp->inv_vx), 0), Jp->nx - 1);
+        const int iy = min(max((int)floorf((qy - Jp->by0) * Jp->inv_vy), 0), Jp->ny - 1);
+        const int iz = min(max((int)floorf((qz - Jp->bz0) * Jp->inv_vz), 0), Jp->nz - 1);
+        const int lab = Jp->labels[((size_t)iz * Jp->ny + iy) * Jp->nx + ix];
+        const float mu = Jp->tables->m[lab][p.eb].x;
+        const float u2 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+        if (u2 * p.mumax < mu) {  // real collision: park until the wave runs interactions
+          p.px = qx; p.py = qy; p.pz = qz;
+          p.lab = lab;
+          state = kCollide;
+        }
+      }
+    }
+    cnt.steps += popc(__ballot(stepped));
+    count_tally(cnt, tallied, code);
   }
   if (job >= 0) flush_counters(cnt, counters + job);
 }
@@ -407,112 +446,118 @@ hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total
 // One thread per (view, pixel): per-label path lengths along the source ->
 // pixel-centre ray by index-stepping traversal (each crossing recomputed from
 // its plane index, no running sums), kept in LDS, then combined with the
-// attenuation of each spectral bin.
-__global__ __launch_bounds__(256) void primary_exact_kernel(McJob J, const float* __restrict__ spec_w, int use_spec,
-                                                            float* __restrict__ out, int n_views) {
-  __shared__ float lens[kMaxLabels][256];
+// attenuation of each spectral bin.  This is a deterministic integral, not MC
+// tracking, and it runs in FP64 end to end: in FP32 the ray set-up from a
+// source tens of cm away moves grazing rays by ~1e-6 cm, which changes short
+// chords by ~1e-4 relative.
+__global__ __launch_bounds__(256) void primary_exact_kernel(PrimaryGeom P, const double* __restrict__ cs64,
+                                                            const uint8_t* __restrict__ labels,
+                                                            const McTables* __restrict__ tables, float e_kev,
+                                                            const float* __restrict__ spec_w, int use_spec,
+                                                            float* __restrict__ out) {
+  __shared__ double lens[kMaxLabels][256];
   const int tid = threadIdx.x;
   const long ray = (long)blockIdx.x * 256 + tid;
-  const long nray = (long)n_views * J.nu * J.nv;
+  const long nray = (long)P.n_views * P.nu * P.nv;
 #pragma unroll
-  for (int l = 0; l < kMaxLabels; ++l) lens[l][tid] = 0.f;
+  for (int l = 0; l < kMaxLabels; ++l) lens[l][tid] = 0.0;
   if (ray >= nray) return;
-  const int k = (int)(ray / ((long)J.nu * J.nv));
-  const int pix = (int)(ray - (long)k * J.nu * J.nv);
-  const int iu = pix / J.nv, iv = pix - iu * J.nv;
-  const float2 cs = J.view_cs[J.view_begin + k];
-  const float yp = ((float)iu + 0.5f - J.un0) * J.u_step;
-  const float zp = ((float)iv + 0.5f - J.vn0) * J.v_step;
-  const float xd = J.sdd - J.sod;
-  const float p0x = -J.sod * cs.x, p0y = -J.sod * cs.y, p0z = 0.f;
-  const float p1x = xd * cs.x - yp * cs.y, p1y = xd * cs.y + yp * cs.x, p1z = zp;
-  float dx = p1x - p0x, dy = p1y - p0y, dz = p1z - p0z;
-  const float L = sqrtf(dx * dx + dy * dy + dz * dz);
-  const float il = 1.0f / L;
-  dx *= il; dy *= il; dz *= il;
-  float t0, t1;
-  if (ray_box(p0x, p0y, p0z, dx, dy, dz, J, t0, t1)) {
-    t0 = fmaxf(t0, 0.f);
-    t1 = fminf(t1, L);
-    if (t1 > t0) {
-      const float ex = fmaf(t0, dx, p0x), ey = fmaf(t0, dy, p0y), ez = fmaf(t0, dz, p0z);
-      const float total = t1 - t0;
-      const float e[3] = {ex, ey, ez}, d[3] = {dx, dy, dz};
-      const float b0[3] = {J.bx0, J.by0, J.bz0};
-      const float vs[3] = {J.vx, J.vy, J.vz};
-      const float ivs[3] = {J.inv_vx, J.inv_vy, J.inv_vz};
-      const int n[3] = {J.nx, J.ny, J.nz};
-      int idx[3], step[3];
-      float sn[3], invd[3];
-      const float sm = fminf(0.5f * total, 1e-4f);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        int i = (int)floorf((fmaf(sm, d[a], e[a]) - b0[a]) * ivs[a]);
-        i = min(max(i, 0), n[a] - 1);
-        idx[a] = i;
-        invd[a] = d[a] != 0.f ? 1.0f / d[a] : 0.f;
-        if (d[a] > 0.f) {
-          step[a] = 1;
-          sn[a] = (fmaf((float)(i + 1), vs[a], b0[a]) - e[a]) * invd[a];
-        } else if (d[a] < 0.f) {
-          step[a] = -1;
-          sn[a] = (fmaf((float)i, vs[a], b0[a]) - e[a]) * invd[a];
-        } else {
-          step[a] = 0;
-          sn[a] = INFINITY;
-        }
-      }
-      float s = 0.f;
-      for (int guard = 0; guard < 4 * (J.nx + J.ny + J.nz) + 8; ++guard) {
-        int a = 0;
-        if (sn[1] < sn[a]) a = 1;
-        if (sn[2] < sn[a]) a = 2;
-        const float se = fminf(sn[a], total);
-        if (se > s) {
-          const int lab = J.labels[((size_t)idx[2] * J.ny + idx[1]) * J.nx + idx[0]];
-          lens[lab][tid] += se - s;
-          s = se;
-        }
-        if (sn[a] >= total) break;
-        idx[a] += step[a];
-        if (idx[a] < 0 || idx[a] >= n[a]) break;
-        sn[a] = (fmaf((float)(step[a] > 0 ? idx[a] + 1 : idx[a]), vs[a], b0[a]) - e[a]) * invd[a];
-      }
+  const int k = (int)(ray / ((long)P.nu * P.nv));
+  const int pix = (int)(ray - (long)k * P.nu * P.nv);
+  const int iu = pix / P.nv, iv = pix - iu * P.nv;
+  const double cb = cs64[2 * k], sb = cs64[2 * k + 1];
+  const double yp = ((double)iu + 0.5 - P.un0) * P.u_step;
+  const double zp = ((double)iv + 0.5 - P.vn0) * P.v_step;
+  const double xd = P.sdd - P.sod;
+  const double p0[3] = {-P.sod * cb, -P.sod * sb, 0.0};
+  double d[3] = {xd * cb - yp * sb - p0[0], xd * sb + yp * cb - p0[1], zp};
+  const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  for (int a = 0; a < 3; ++a) d[a] /= L;
+  double t0 = -INFINITY, t1 = INFINITY;
+  bool hit = true;
+  for (int a = 0; a < 3; ++a) {
+    if (d[a] != 0.0) {
+      double ta = (P.b0[a] - p0[a]) / d[a], tb = (P.b1[a] - p0[a]) / d[a];
+      if (ta > tb) { const double tt = ta; ta = tb; tb = tt; }
+      t0 = fmax(t0, ta);
+      t1 = fmin(t1, tb);
+    } else if (p0[a] < P.b0[a] || p0[a] >= P.b1[a]) {
+      hit = false;
     }
   }
-  float result;
+  t0 = fmax(t0, 0.0);
+  t1 = fmin(t1, L);
+  if (hit && t1 > t0) {
+    int idx[3], step[3];
+    double sn[3];
+    const double tm = t0 + fmin(0.5 * (t1 - t0), 1e-9);
+    for (int a = 0; a < 3; ++a) {
+      int i = (int)floor((p0[a] + tm * d[a] - P.b0[a]) / P.vox[a]);
+      i = min(max(i, 0), P.n[a] - 1);
+      idx[a] = i;
+      if (d[a] > 0.0) {
+        step[a] = 1;
+        sn[a] = (P.b0[a] + (i + 1) * P.vox[a] - p0[a]) / d[a];
+      } else if (d[a] < 0.0) {
+        step[a] = -1;
+        sn[a] = (P.b0[a] + i * P.vox[a] - p0[a]) / d[a];
+      } else {
+        step[a] = 0;
+        sn[a] = INFINITY;
+      }
+    }
+    double t = t0;
+    for (int guard = 0; guard < 4 * (P.n[0] + P.n[1] + P.n[2]) + 8; ++guard) {
+      int a = 0;
+      if (sn[1] < sn[a]) a = 1;
+      if (sn[2] < sn[a]) a = 2;
+      const double te = fmin(sn[a], t1);
+      if (te > t) {
+        const int lab = labels[((size_t)idx[2] * P.n[1] + idx[1]) * P.n[0] + idx[0]];
+        lens[lab][tid] += te - t;
+        t = te;
+      }
+      if (sn[a] >= t1) break;
+      idx[a] += step[a];
+      if (idx[a] < 0 || idx[a] >= P.n[a]) break;
+      sn[a] = (P.b0[a] + (step[a] > 0 ? idx[a] + 1 : idx[a]) * P.vox[a] - p0[a]) / d[a];
+    }
+  }
+  double result;
   if (!use_spec) {
-    const int eb = ebin_of(J.e_kev);
-    float acc = 0.f;
+    const int eb = ebin_of(e_kev);
+    double acc = 0.0;
     for (int l = 0; l < kMaxLabels; ++l) {
-      const float len = lens[l][tid];
-      if (len != 0.f) acc = fmaf(len, J.tables->m[l][eb].x, acc);
+      const double len = lens[l][tid];
+      if (len != 0.0) acc += len * (double)tables->m[l][eb].x;
     }
     result = acc;
   } else {
-    float num = 0.f, den = 0.f;
+    double num = 0.0, den = 0.0;
     for (int eb = 1; eb < kRows; ++eb) {
-      const float w = spec_w[eb];
-      if (w <= 0.f) continue;
-      float le = 0.f;
+      const double w = spec_w[eb];
+      if (w <= 0.0) continue;
+      double le = 0.0;
       for (int l = 0; l < kMaxLabels; ++l) {
-        const float len = lens[l][tid];
-        if (len != 0.f) le = fmaf(len, J.tables->m[l][eb].x, le);
+        const double len = lens[l][tid];
+        if (len != 0.0) le += len * (double)tables->m[l][eb].x;
       }
-      num = fmaf(w, expf(-le), num);
+      num += w * exp(-le);
       den += w;
     }
-    result = -logf(num / den);
+    result = -log(num / den);
   }
-  out[ray] = result;
+  out[ray] = (float)result;
 }
 
-hipError_t launch_primary_exact(const McJob& job, const float* spec_w, int n_spec, float* out, hipStream_t s) {
-  const long nray = (long)job.n_views_sim * job.nu * job.nv;
+hipError_t launch_primary_exact(const PrimaryGeom& P, const double* cs64, const uint8_t* labels,
+                                const McTables* tables, float e_kev, const float* spec_w, int use_spec, float* out,
+                                hipStream_t s) {
+  const long nray = (long)P.n_views * P.nu * P.nv;
   if (nray <= 0) return hipSuccess;
   const long blocks = (nray + 255) / 256;
-  hipLaunchKernelGGL(primary_exact_kernel, dim3((unsigned)blocks), dim3(256), 0, s, job, spec_w, n_spec > 0 ? 1 : 0,
-                     out, job.n_views_sim);
+  hipLaunchKernelGGL(primary_exact_kernel, dim3((unsigned)blocks), dim3(256), 0, s, P, cs64, labels, tables, e_kev,
+                     spec_w, use_spec, out);
   return hipGetLastError();
 }
 

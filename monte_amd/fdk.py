@@ -79,8 +79,24 @@ def fdk_graph(proj: torch.Tensor, geom: Geometry, out: torch.Tensor, ctx: Contex
     if out.dtype != torch.float32 or not out.is_contiguous() or out.numel() != xb.shape[0] * geom.nx * geom.ny * geom.nz:
         raise ValueError("fdk_graph: out must be a contiguous float32 tensor of the volume batch size")
     g = geom.c()
-    ctx.call("monte_fdk_graph", C.byref(g), _ptr(xb), _ptr(out), xb.shape[0])
+    # capture needs a created stream: run on a side stream ordered against the caller's
+    cur = torch.cuda.current_stream(proj.device)
+    side = _graph_stream(proj.device)
+    side.wait_stream(cur)
+    with torch.cuda.stream(side):
+        ctx.call("monte_fdk_graph", C.byref(g), _ptr(xb), _ptr(out), xb.shape[0])
+    cur.wait_stream(side)
     return out
+
+
+_graph_streams: dict = {}
+
+
+def _graph_stream(device):
+    key = device.index or 0
+    if key not in _graph_streams:
+        _graph_streams[key] = torch.cuda.Stream(device)
+    return _graph_streams[key]
 
 
 class FDKFunction(torch.autograd.Function):

@@ -1,0 +1,129 @@
+"""The FDK adjoint and its torch.autograd binding on the GPU.
+
+* <A x, y> = <x, A^T y> to 1e-5 relative (sums in FP64) for the whole operator
+  and for each stage, in both presets;
+* A^T against the oracle's explicit FP64 adjoint;
+* gradients of a loss on the FDK volume w.r.t. the projections against FP64
+  central finite differences of the oracle's FDK, at 33^3 / 64^2 / 30 views.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_geom, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _dot(a, b):
+    return float(np.sum(np.asarray(a, np.float64) * np.asarray(b, np.float64)))
+
+
+def _geoms():
+    from monte_amd import geometry as G
+
+    std = G.fd_check().replace(n_views=20, nu=48, nv=40, u_off=0.05, v_off=-0.04, ox=0.03, oz=0.05)
+    shepp = std.replace(filter=1, n_views=16)
+    compat = G.bp3d20_65(n_views=24).replace(nx=64, ny=64, nz=64, roi_x0=20, roi_x1=44, vx=0.4, vy=0.4, vz=0.4,
+                                             mask_r=-1.0)
+    compat_mask = compat.replace(mask_r=25.0)
+    return {"std": std, "shepp": shepp, "bp3d20": compat, "bp3d20_mask": compat_mask}
+
+
+@pytest.mark.parametrize("name", ["std", "shepp", "bp3d20", "bp3d20_mask"])
+def test_dot_product_identity(ctx, name):
+    from monte_amd.fdk import (fdk, fdk_adjoint, fdk_backproject, fdk_filter, fdk_filter_adjoint,
+                               fdk_project)
+
+    g = _geoms()[name]
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(g.proj_shape).astype(np.float32)
+    y = rng.standard_normal(g.vol_shape).astype(np.float32)
+    if g.preset == 1:  # only the ROI is reconstructed
+        y[:, :, : g.roi_x0] = 0
+        y[:, :, g.roi_x1:] = 0
+    ax = fdk(_t(x), g, ctx).cpu().numpy()
+    aty = fdk_adjoint(_t(y), g, ctx).cpu().numpy()
+    lhs, rhs = _dot(ax, y), _dot(x, aty)
+    print(name, lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    # stages
+    f = rng.standard_normal(g.filt_shape).astype(np.float32)
+    b1, b2 = _dot(fdk_backproject(_t(f), g, ctx).cpu().numpy(), y), _dot(f, fdk_project(_t(y), g, ctx).cpu().numpy())
+    assert abs(b1 - b2) <= 1e-5 * abs(b1)
+    f1, f2 = _dot(fdk_filter(_t(x), g, ctx).cpu().numpy(), f), _dot(x, fdk_filter_adjoint(_t(f), g, ctx).cpu().numpy())
+    assert abs(f1 - f2) <= 1e-5 * abs(f1)
+
+
+def test_adjoint_matches_oracle(ctx, oracle):
+    from monte_amd.fdk import fdk_adjoint
+
+    g = _geoms()["std"]
+    rng = np.random.default_rng(5)
+    y = rng.standard_normal(g.vol_shape).astype(np.float32)
+    got = fdk_adjoint(_t(y), g, ctx).cpu().numpy()
+    ref = oracle.fdk_std_adjoint(oracle_geom(oracle, g), y)
+    print("adjoint rel_max", rel_max(got, ref), "rel_l2", rel_l2(got, ref))
+    assert rel_max(got, ref) < 1e-5
+    assert rel_l2(got, ref) < 1e-5
+
+
+def test_autograd_matches_fp64_finite_differences(ctx, oracle):
+    """d/dp of L = sum(g * FDK(p)) at 33^3 / 64^2 / 30 views."""
+    import torch
+
+    from monte_amd import geometry as G
+    from monte_amd.fdk import fdk_autograd
+
+    geom = G.fd_check()
+    og = oracle_geom(oracle, geom)
+    rng = np.random.default_rng(8)
+    p = rng.standard_normal(geom.proj_shape)
+    gw = rng.standard_normal(geom.vol_shape)
+    x = _t(p.astype(np.float32)).requires_grad_(True)
+    vol = fdk_autograd(x, geom, ctx)
+    (vol * _t(gw.astype(np.float32))).sum().backward()
+    grad = x.grad.cpu().numpy()
+    # FP64 central differences of the oracle FDK at a sample of entries
+    eps = 1e-3
+    idx = [tuple(int(rng.integers(0, s)) for s in geom.proj_shape) for _ in range(12)]
+    # entries whose ray meets the volume, so the derivative is not trivially zero
+    idx += [(k, 32, 32) for k in (0, 7, 29)]
+    fd = []
+    for i in idx:
+        e = np.zeros(geom.proj_shape)
+        e[i] = eps
+        lp = np.sum(gw * oracle.fdk_std(og, p + e))
+        lm = np.sum(gw * oracle.fdk_std(og, p - e))
+        fd.append((lp - lm) / (2 * eps))
+    fd = np.array(fd)
+    got = np.array([grad[i] for i in idx])
+    scale = np.abs(grad).max()
+    print("fd", fd[-3:], "gpu", got[-3:], "max|diff|/max|grad|", np.abs(got - fd).max() / scale)
+    assert np.abs(got - fd).max() <= 1e-5 * scale
+    # and the whole gradient against the explicit FP64 adjoint
+    ref = oracle.fdk_std_adjoint(og, gw)
+    assert rel_max(grad, ref) < 1e-5
+
+
+def test_autograd_batch_and_graph_of_ops(ctx):
+    """Gradients flow through a batch and through torch ops around the FDK."""
+    import torch
+
+    from monte_amd import geometry as G
+    from monte_amd.fdk import fdk_adjoint, fdk_autograd
+
+    geom = G.fd_check().replace(n_views=10)
+    rng = np.random.default_rng(9)
+    x = _t(rng.standard_normal((3, *geom.proj_shape)).astype(np.float32)).requires_grad_(True)
+    target = _t(rng.standard_normal((3, *geom.vol_shape)).astype(np.float32))
+    vol = fdk_autograd(x * 2.0, geom, ctx)
+    loss = 0.5 * ((vol - target) ** 2).sum()
+    loss.backward()
+    expect = 2.0 * fdk_adjoint((vol - target).detach().contiguous(), geom, ctx)
+    assert torch.allclose(x.grad, expect, rtol=1e-6, atol=1e-6 * expect.abs().max().item())
