@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--photons", type=float, default=1e8, help="photons per view")
     ap.add_argument("--views", type=int, default=360)
     ap.add_argument("--batch", type=int, default=8, help="stacks in the differentiable-FDK step")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline budget (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=24.0, help="CPU-baseline budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--autotune", type=int, default=1)
     return ap.parse_args()
@@ -179,7 +179,9 @@ def main():
 
 
 def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
-    """Oracle (FP64 C restatement) on the host cores, on a bounded sample."""
+    """Oracle (FP64 C restatement) on the host cores, on a bounded sample: a
+    short probe sets the rate, then one run sized to about half the budget each
+    for MC and FDK."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -192,34 +194,35 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
     oph = O.phantom(ph.labels, *ph.voxel, materials=ph.materials)
     tb = O.tables(tables.reference_tables())
     cdf = tables.spectrum_cdf_f32(spec)
-    budget = a.cpu_seconds / 2
-    # MC: grow the photon sample until it takes a measurable fraction of the budget
-    n = 20000
-    while True:
+    target = a.cpu_seconds / 2
+
+    def mc_time(n):
         t0 = time.perf_counter()
         O.mc_run(og, oph, tb, n, spectrum_cdf=cdf, coherent_mode=1, views=(0, 2), nthreads=nthreads)
-        dt = time.perf_counter() - t0
-        if dt > budget / 4 or n > 2e8:
-            break
-        n = int(n * max(2.0, min(8.0, budget / 4 / max(dt, 1e-3))))
+        return time.perf_counter() - t0
+
+    probe = 200_000
+    rate = 2 * probe / max(mc_time(probe), 1e-3)
+    n = max(probe, int(rate * target / 2))
+    dt = mc_time(n)
     mc = {"value": 2 * n / dt, "unit": "histories/s", "cores": nthreads, "kind": "port",
-          "sample": f"oracle MC (FP64, same physics and streams) on 2 views x {n} photons of the workload, "
-                    f"{dt:.1f} s", "gpu_over_cpu": gpu_hps / (2 * n / dt)}
-    # FDK: full filter + backprojection of a subset of views onto the full volume
-    nv = 2
-    while True:
+          "sample": f"oracle MC (FP64, same physics and Philox streams) on 2 views x {n} photons of the "
+                    f"workload, {dt:.1f} s", "gpu_over_cpu": gpu_hps / (2 * n / dt)}
+
+    def fdk_time(nv):
         g = O.geom(**{**geom.as_dict(), "n_views": nv, "dangle": geom.c().dangle})
         proj = np.random.default_rng(0).standard_normal((nv, geom.nu, geom.nv)).astype(np.float32)
         t0 = time.perf_counter()
         O.fdk_std(g, proj, nthreads)
-        dt = time.perf_counter() - t0
-        if dt > budget / 4 or nv >= geom.n_views:
-            break
-        nv = min(geom.n_views, int(nv * max(2.0, min(8.0, budget / 4 / max(dt, 1e-3)))))
+        return time.perf_counter() - t0
+
+    per_view = fdk_time(1)
+    nv = int(min(geom.n_views, max(2, target / max(per_view, 1e-3))))
+    dt = fdk_time(nv)
     ups = nv * geom.nx * geom.ny * geom.nz / dt
     fdk = {"value": ups / 1e9, "unit": "GUPS", "cores": nthreads, "kind": "port",
-           "sample": f"oracle FDK (FP64) of {nv} views onto the full 325^3 volume, {dt:.1f} s",
-           "gpu_over_cpu": gpu_ups / ups}
+           "sample": f"oracle FDK (FP64: weight, direct-convolution filter, backprojection) of {nv} views "
+                     f"onto the full {geom.nx}^3 volume, {dt:.1f} s", "gpu_over_cpu": gpu_ups / ups}
     return mc, fdk
 
 

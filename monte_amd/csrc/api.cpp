@@ -42,7 +42,7 @@ void* workspace(monte_ctx* c, int slot, size_t bytes) {
 namespace {
 
 constexpr int kSlotFwd = 0, kSlotAdj = 1, kSlotMc = 2, kSlotConst = 3, kSlotStageA = 4, kSlotStageB = 5,
-              kSlotTuneA = 6, kSlotTuneB = 7;
+              kSlotTuneA = 6, kSlotTuneB = 7, kSlotVolT = 8;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -52,6 +52,7 @@ int check_geom(monte_ctx* c, const monte_geom* g, bool need_volume) {
     return fail(c, MONTE_ERR_INVALID, "unknown preset");
   if (g->n_views < 1 || g->nu < 1 || g->nv < 1) return fail(c, MONTE_ERR_INVALID, "detector/view counts must be >= 1");
   if (g->nu > 1024) return fail(c, MONTE_ERR_UNSUPPORTED, "detector rows longer than 1024 pixels are not supported");
+  if (g->nv > 1536) return fail(c, MONTE_ERR_UNSUPPORTED, "detectors taller than 1536 pixels are not supported");
   if (!(g->sod > 0) || !(g->sdd > g->sod)) return fail(c, MONTE_ERR_INVALID, "need 0 < sod < sdd");
   if (!(g->du > 0) || !(g->dv > 0)) return fail(c, MONTE_ERR_INVALID, "detector pitch must be > 0");
   if (g->preset == MONTE_PRESET_BP3D20 && g->step_deg < 1) return fail(c, MONTE_ERR_INVALID, "step_deg must be >= 1");
@@ -177,11 +178,13 @@ int run_project_padded(monte_ctx* c, const monte_geom& g, const float* vol, floa
   int rc = ensure_constants(c, g, &taps, &views);
   if (rc) return rc;
   const BpGrid grid = compute_bp_grid(g);
-  MONTE_HIP(c, launch_fill(pad, 0.f, padded_elems(g), c->stream));
+  float* volT = (float*)workspace(c, kSlotVolT, vol_elems(g) * sizeof(float));
+  if (!volT) return fail(c, MONTE_ERR_NOMEM, "transposed volume allocation failed");
+  MONTE_HIP(c, launch_transpose_zx(vol, volT, g.nx, g.ny, g.nz, c->stream));
   const int chunk = c->tuning.bp_view_chunk > 0 ? c->tuning.bp_view_chunk : g.n_views;
   for (int v0 = 0; v0 < g.n_views; v0 += chunk) {
     const int n = std::min(chunk, g.n_views - v0);
-    MONTE_HIP(c, launch_project(vol, views, v0, n, grid, pad, c->tuning, c->stream));
+    MONTE_HIP(c, launch_project(volT, views, v0, n, grid, pad, c->tuning, c->stream));
   }
   if (g.preset == MONTE_PRESET_BP3D20)
     MONTE_HIP(c, launch_fold_pads(pad, g.n_views, g.nu, g.nv, grid.pitch, grid.rows, c->stream));

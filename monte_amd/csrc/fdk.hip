@@ -326,22 +326,20 @@ __device__ inline int xcd_tile(int b, int nblocks) {
   return xcd * q + (xcd < r ? xcd : r) + j;
 }
 
+
 // Per-column quantities of one view, shared by the backprojector and its
 // transpose so both evaluate bit-identical weights.
-struct ColumnView {
+struct ColumnCore {
   int iu;       // u pixel index of the left tap (sample window clamped)
   float fu;     // u fraction
-  int ivb;      // integer part of sv at the base voxel zb
-  float fvb;    // its fraction
-  float dsv;    // sv increment per voxel step along z
-  float flo, fhi;  // v window relative to ivb
   float w;      // distance weight x angular factor
   bool ok;      // su inside the window
+  double kv;    // d(sv)/dZ = svk / U
+  float dsv;    // sv increment per voxel step along z
 };
 
-__device__ inline ColumnView column_view(const BpView& c, double X, double Y, float Xf, float Yf, double Zb,
-                                         const BpGrid& g) {
-  ColumnView r;
+__device__ inline ColumnCore column_core(const BpView& c, double X, double Y, float Xf, float Yf, const BpGrid& g) {
+  ColumnCore r;
   const double U = fma(c.cx, X, fma(c.cy, Y, c.c0));
   const double inv = 1.0 / U;
   double su = fma(c.suk * fma(c.ax, X, c.ay * Y), inv, c.su0);
@@ -350,15 +348,8 @@ __device__ inline ColumnView column_view(const BpView& c, double X, double Y, fl
   const double fu0 = floor(su);
   r.iu = (int)fu0;
   r.fu = (float)(su - fu0);
-  const double k = c.svk * inv;
-  double sv = fma(k, Zb, c.sv0);
-  sv = fmin(fmax(sv, -1.0e6), 1.0e6);
-  const double fv0 = floor(sv);
-  r.ivb = (int)fv0;
-  r.fvb = (float)(sv - fv0);
-  r.dsv = (float)(k * g.gsz);
-  r.flo = (float)(g.lo_v - fv0);
-  r.fhi = (float)(g.hi_v - fv0);
+  r.kv = c.svk * inv;
+  r.dsv = (float)(r.kv * g.gsz);
   // sign test in unfused FP64 (as recon/bp3d20_325.cpp:145,151 evaluates it)
   const double gp = __dadd_rn(__dmul_rn(X, c.gx), __dmul_rn(Y, c.gy));
   const float hp = fabsf(fmaf(c.hx, Xf, c.hy * Yf));
@@ -368,15 +359,45 @@ __device__ inline ColumnView column_view(const BpView& c, double X, double Y, fl
   return r;
 }
 
-// Sample position of voxel j of the chunk: row index and fraction, and whether
-// it lies in the v window.  (Identical in both directions.)
-__device__ inline void column_row(const ColumnView& cv, int j, int& row, float& fv, bool& okv) {
-  float f = fmaf((float)j, cv.dsv, cv.fvb);
-  okv = f >= cv.flo && f <= cv.fhi;
-  f = fminf(fmaxf(f, cv.flo), cv.fhi);
+// v coordinate at a 16-aligned base voxel: integer part and FP32 fraction.
+struct ChunkBase {
+  int ivb;
+  float fvb;
+  float flo, fhi;  // v window relative to ivb
+};
+
+// The v coordinate of voxel z is formed from a base taken at the 16-aligned z
+// at or below it (zb), whatever the thread's z-chunk, so every chunking and
+// tile shape evaluates the same floating-point expression for each voxel and
+// the volume is bitwise independent of the launch tuning.
+constexpr int kZBase = 16;
+
+__device__ inline double base_z(const BpGrid& g, int zb) {
+  // recon/bp3d20_325.cpp:109 forms the coordinate as start - index * step, unfused
+  return __dadd_rn(g.gz0, __dmul_rn((double)zb, g.gsz));
+}
+
+__device__ inline ChunkBase chunk_base(const ColumnCore& cc, const BpView& c, double Zb, const BpGrid& g) {
+  ChunkBase b;
+  double sv = fma(cc.kv, Zb, c.sv0);
+  sv = fmin(fmax(sv, -1.0e6), 1.0e6);
+  const double fv0 = floor(sv);
+  b.ivb = (int)fv0;
+  b.fvb = (float)(sv - fv0);
+  b.flo = (float)(g.lo_v - fv0);
+  b.fhi = (float)(g.hi_v - fv0);
+  return b;
+}
+
+// Sample position of voxel zb + j: row index and fraction, and whether it
+// lies in the v window.  (Identical in both directions.)
+__device__ inline void chunk_row(const ColumnCore& cc, const ChunkBase& b, int j, int& row, float& fv, bool& okv) {
+  float f = fmaf((float)j, cc.dsv, b.fvb);
+  okv = f >= b.flo && f <= b.fhi;
+  f = fminf(fmaxf(f, b.flo), b.fhi);
   const float fl = floorf(f);
   fv = f - fl;
-  row = cv.ivb + (int)fl;
+  row = b.ivb + (int)fl;
 }
 
 __device__ inline bool in_mask(const BpGrid& g, int x, int y, int z) {
@@ -385,78 +406,60 @@ __device__ inline bool in_mask(const BpGrid& g, int x, int y, int z) {
   return (float)(dz * dz + dy * dy + dx * dx) <= g.mask_r2;
 }
 
-// The v coordinate of voxel z is formed from a base taken at the 16-aligned z
-// at or below it (zb), whatever the thread's z-chunk, so every chunking and
-// tile shape evaluates the same floating-point expression for each voxel and
-// the volume is bitwise independent of the launch tuning.
-constexpr int kZBase = 16;
-
-struct ColumnPos {
-  int x, y, z0;
-  int joff;           // z0 - zb
-  double X, Y, Zb;    // Zb: Z of the base voxel zb
-  float Xf, Yf;
-};
-
-__device__ inline bool column_pos(const BpGrid& g, int tile_x, int tiles_x, int tiles_y, int zc_size,
-                                  ColumnPos& p) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int per_z = tiles_x * tiles_y;
-  const int zc = t / per_z;
-  const int rem = t - zc * per_z;
-  const int ty_t = rem / tiles_x, tx_t = rem - ty_t * tiles_x;
-  const int tile_y = 256 / tile_x;
-  p.x = g.x0 + tx_t * tile_x + (int)(threadIdx.x % tile_x);
-  p.y = ty_t * tile_y + (int)(threadIdx.x / tile_x);
-  p.z0 = zc * zc_size;
-  const int zb = (p.z0 / kZBase) * kZBase;
-  p.joff = p.z0 - zb;
-  // recon/bp3d20_325.cpp:107-109 form the coordinate as start + index * step, unfused
-  p.X = __dadd_rn(g.gx0, __dmul_rn((double)p.x, g.gsx));
-  p.Y = __dadd_rn(g.gy0, __dmul_rn((double)p.y, g.gsy));
-  p.Zb = __dadd_rn(g.gz0, __dmul_rn((double)zb, g.gsz));
-  p.Xf = (float)p.X;
-  p.Yf = (float)p.Y;
-  return p.x < g.x1 && p.y < g.ny;
-}
+// recon/bp3d20_325.cpp:107-108 form X and Y as start + index * step, unfused
+__device__ inline double voxel_x(const BpGrid& g, int x) { return __dadd_rn(g.gx0, __dmul_rn((double)x, g.gsx)); }
+__device__ inline double voxel_y(const BpGrid& g, int y) { return __dadd_rn(g.gy0, __dmul_rn((double)y, g.gsy)); }
 
 template <int ZC>
 __global__ __launch_bounds__(256) void backproject_kernel(const float* __restrict__ filt,
                                                           const BpView* __restrict__ views, int view0, int nviews,
                                                           BpGrid g, float* __restrict__ vol, int accumulate,
                                                           int tile_x, int tiles_x, int tiles_y) {
-  ColumnPos cp;
-  if (!column_pos(g, tile_x, tiles_x, tiles_y, ZC, cp)) return;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int per_z = tiles_x * tiles_y;
+  const int zc = t / per_z;
+  const int rem = t - zc * per_z;
+  const int ty_t = rem / tiles_x, tx_t = rem - ty_t * tiles_x;
+  const int tile_y = 256 / tile_x;
+  const int x = g.x0 + tx_t * tile_x + (int)(threadIdx.x % tile_x);
+  const int y = ty_t * tile_y + (int)(threadIdx.x / tile_x);
+  const int z0 = zc * ZC;
+  if (x >= g.x1 || y >= g.ny) return;
+  const int zb = (z0 / kZBase) * kZBase, joff = z0 - zb;
+  const double X = voxel_x(g, x), Y = voxel_y(g, y), Zb = base_z(g, zb);
+  const float Xf = (float)X, Yf = (float)Y;
   bool live[ZC];
 #pragma unroll
-  for (int j = 0; j < ZC; ++j) live[j] = (cp.z0 + j) < g.nz && in_mask(g, cp.x, cp.y, cp.z0 + j);
+  for (int j = 0; j < ZC; ++j) live[j] = (z0 + j) < g.nz && in_mask(g, x, y, z0 + j);
   float acc[ZC];
 #pragma unroll
   for (int j = 0; j < ZC; ++j) acc[j] = 0.f;
 
   const size_t view_stride = (size_t)g.rows * g.pitch;
   for (int k = view0; k < view0 + nviews; ++k) {
-    const ColumnView cv = column_view(views[k], cp.X, cp.Y, cp.Xf, cp.Yf, cp.Zb, g);
-    const float* base = filt + (size_t)k * view_stride + (cv.iu + kPadLo);
-    const float wu = cv.ok ? cv.w : 0.f;
+    const BpView& c = views[k];
+    const ColumnCore cc = column_core(c, X, Y, Xf, Yf, g);
+    const ChunkBase cb = chunk_base(cc, c, Zb, g);
+    const float* base = filt + (size_t)k * view_stride + (cc.iu + kPadLo);
+    const float wu = cc.ok ? cc.w : 0.f;
 #pragma unroll
     for (int j = 0; j < ZC; ++j) {
       int row;
       float fv;
       bool okv;
-      column_row(cv, cp.joff + j, row, fv, okv);
+      chunk_row(cc, cb, joff + j, row, fv, okv);
       const float* p = base + (size_t)(row + kPadLo) * g.pitch;
       const float p00 = p[0], p01 = p[1], p10 = p[g.pitch], p11 = p[g.pitch + 1];
-      const float a = fmaf(cv.fu, p01 - p00, p00);
-      const float b = fmaf(cv.fu, p11 - p10, p10);
+      const float a = fmaf(cc.fu, p01 - p00, p00);
+      const float b = fmaf(cc.fu, p11 - p10, p10);
       const float val = fmaf(fv, b - a, a);
       acc[j] = fmaf((okv && live[j]) ? wu : 0.f, val, acc[j]);
     }
   }
 #pragma unroll
   for (int j = 0; j < ZC; ++j) {
-    if (cp.z0 + j >= g.nz) break;
-    float* o = vol + ((size_t)(cp.z0 + j) * g.ny + cp.y) * g.nx + cp.x;
+    if (z0 + j >= g.nz) break;
+    float* o = vol + ((size_t)(z0 + j) * g.ny + y) * g.nx + x;
     *o = accumulate ? *o + acc[j] : acc[j];
   }
 }
@@ -493,139 +496,160 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 }
 
 // ------------------------------------------- transposed backprojection
-// The matched projector.  Every (voxel, view) pair deposits vol * w *
-// bilinear weight on the four taps the backprojector gathers from, with the
-// same column_view / column_row arithmetic, so <BP q, x> == <q, BP^T x> up to
-// the order of the sums.  A block owns a tile of voxel columns x ZT voxels
-// along z (values kept in registers) and walks the views: for each view it
-// finds the tile's detector footprint (block min/max of the taps touched),
-// accumulates into a privatised copy of that footprint in LDS with LDS
-// atomics, then adds the footprint to the global stack with one global atomic
-// per touched pixel.  Footprints larger than the LDS patch fall back to global
-// atomics for that view.  Taps on guard cells are dropped or folded by the
-// callers.
-constexpr int kPatch = 8192;  // LDS floats per block
+// The matched projector, written as a gather so no global atomics are needed.
+// A block owns a strip of kStrip padded detector columns (all rows) of one
+// view, held in LDS.  It enumerates the voxel columns whose left tap can land
+// in the strip: for each slab of the volume across the view's rays (x-slabs
+// when |cos b| >= |sin b|, y-slabs otherwise) the projective map su(X,Y) is
+// inverted at the strip's two edges, giving the range of columns on that slab
+// (widened by one voxel each side; every column is then tested exactly).  A
+// wave takes one column at a time and its lanes walk z over a z-fastest copy
+// of the volume, 64 voxels per pass, with the backprojector's own per-voxel
+// arithmetic (same column core, same 16-aligned bases), so every weight equals
+// the backprojector's bit for bit.  Each voxel deposits its (up to four) taps
+// that fall inside the strip with LDS atomics; the strip is then stored once.
+constexpr int kStrip = 8;
+constexpr int kStripLd = kStrip + 1;  // padded LDS row: consecutive rows fall in different banks
+constexpr int kPassBatch = 8;         // 64-voxel passes whose loads are issued together
 
-__device__ inline int wave_min_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ inline int wave_max_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-template <int ZT>
-__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ vol,
-                                                      const BpView* __restrict__ views, int view0, int nviews,
-                                                      BpGrid g, float* __restrict__ filt, int tile_x, int tiles_x,
-                                                      int tiles_y) {
-  __shared__ float patch[kPatch];
-  __shared__ int bounds[4];  // umin, umax, vmin, vmax over the block
-  ColumnPos cp;
-  const bool col_ok = column_pos(g, tile_x, tiles_x, tiles_y, ZT, cp);
-  float val[ZT];
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < ZT; ++j) {
-    const int z = cp.z0 + j;
-    val[j] = (col_ok && z < g.nz && in_mask(g, cp.x, cp.y, z)) ? vol[((size_t)z * g.ny + cp.y) * g.nx + cp.x] : 0.f;
-    any = any || val[j] != 0.f;
+__device__ inline void slab_range(double s_lo, double s_hi, const BpView& c, double coord, bool x_slabs,
+                                  double o0, double os, int n, int& a, int& b) {
+  // invert su = su0 + suk (ax X + ay Y) / (cx X + cy Y + c0) for the other coordinate
+  double lo = 1e300, hi = -1e300;
+  bool whole = false;
+  for (int e = 0; e < 2; ++e) {
+    const double q = ((e ? s_hi : s_lo) - c.su0) / c.suk;
+    double num, den;
+    if (x_slabs) {  // X fixed, solve Y
+      num = q * (c.cx * coord + c.c0) - c.ax * coord;
+      den = c.ay - q * c.cy;
+    } else {        // Y fixed, solve X
+      num = q * (c.cy * coord + c.c0) - c.ay * coord;
+      den = c.ax - q * c.cx;
+    }
+    if (fabs(den) < 1e-12) { whole = true; break; }
+    const double idx = (num / den - o0) / os;  // fractional voxel index
+    lo = fmin(lo, idx);
+    hi = fmax(hi, idx);
   }
-  const int tid = threadIdx.x;
-  const size_t view_stride = (size_t)g.rows * g.pitch;
-  for (int k = view0; k < view0 + nviews; ++k) {
-    ColumnView cv{};
-    int umin = 1 << 30, umax = -(1 << 30), vmin = 1 << 30, vmax = -(1 << 30);
-    const bool act = col_ok && any;
-    if (act) {
-      cv = column_view(views[k], cp.X, cp.Y, cp.Xf, cp.Yf, cp.Zb, g);
-      if (cv.ok) {
-        umin = cv.iu;
-        umax = cv.iu + 1;
-        int r0, r1;
-        float f;
-        bool o;
-        column_row(cv, cp.joff, r0, f, o);
-        column_row(cv, cp.joff + ZT - 1, r1, f, o);
-        vmin = min(r0, r1);
-        vmax = max(r0, r1) + 1;
-      }
+  if (whole) { a = 0; b = n - 1; return; }
+  a = (int)fmax(floor(lo) - 1.0, 0.0);
+  b = (int)fmin(ceil(hi) + 1.0, (double)(n - 1));
+}
+
+__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ volT,
+                                                      const BpView* __restrict__ views, int view0, int n_strips,
+                                                      BpGrid g, float* __restrict__ filt) {
+  extern __shared__ float strip[];  // [rows][kStripLd]
+  const int k = view0 + blockIdx.x / n_strips;
+  const int st = blockIdx.x - (k - view0) * n_strips;
+  const int pu0 = st * kStrip;       // first padded column of the strip
+  const int u_lo = pu0 - kPadLo;     // first detector column index owned
+  const int u_hi = u_lo + kStrip - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < g.rows * kStripLd; i += 256) strip[i] = 0.f;
+  __syncthreads();
+
+  const BpView c = views[k];  // held in registers: no constant reloads inside the loops
+  const bool x_slabs = fabs(c.cx) >= fabs(c.cy);
+  // columns whose left tap iu satisfies u_lo - 1 <= iu <= u_hi, i.e. su in [u_lo - 1, u_hi + 1)
+  const double s_lo = (double)(u_lo - 1) - 0.5, s_hi = (double)(u_hi + 1) + 0.5;
+  const int n_slabs = x_slabs ? (g.x1 - g.x0) : g.ny;
+  const int npass = (g.nz + 63) / 64;
+  const int grp = lane & ~(kZBase - 1), jj = lane & (kZBase - 1);
+  for (int sl = wave; sl < n_slabs; sl += 4) {
+    int a, b;
+    int xs = 0, ys = 0;
+    if (x_slabs) {
+      xs = g.x0 + sl;
+      slab_range(s_lo, s_hi, c, voxel_x(g, xs), true, g.gy0, g.gsy, g.ny, a, b);
+    } else {
+      ys = sl;
+      slab_range(s_lo, s_hi, c, voxel_y(g, ys), false, g.gx0, g.gsx, g.nx, a, b);
+      a = max(a, g.x0);
+      b = min(b, g.x1 - 1);
     }
-    umin = wave_min_i(umin); umax = wave_max_i(umax);
-    vmin = wave_min_i(vmin); vmax = wave_max_i(vmax);
-    if (tid == 0) { bounds[0] = 1 << 30; bounds[1] = -(1 << 30); bounds[2] = 1 << 30; bounds[3] = -(1 << 30); }
-    __syncthreads();
-    if ((tid & 63) == 0) {
-      atomicMin(&bounds[0], umin); atomicMax(&bounds[1], umax);
-      atomicMin(&bounds[2], vmin); atomicMax(&bounds[3], vmax);
-    }
-    __syncthreads();
-    const int u0 = bounds[0], v0 = bounds[2];
-    const int fu = bounds[1] - u0 + 1, fvn = bounds[3] - v0 + 1;
-    if (fu <= 0 || fvn <= 0) {  // nothing of this tile lands on the detector
-      __syncthreads();
-      continue;
-    }
-    float* out = filt + (size_t)k * view_stride;
-    const bool lds = (long)fu * fvn <= kPatch;
-    if (lds)
-      for (int i = tid; i < fu * fvn; i += 256) patch[i] = 0.f;
-    __syncthreads();
-    if (act && cv.ok) {
+    for (int o = a; o <= b; ++o) {
+      const int x = x_slabs ? xs : o, y = x_slabs ? o : ys;
+      const double X = voxel_x(g, x), Y = voxel_y(g, y);
+      const ColumnCore cc = column_core(c, X, Y, (float)X, (float)Y, g);
+      if (!cc.ok || cc.iu < u_lo - 1 || cc.iu > u_hi) continue;  // wave-uniform
+      const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap, may be -1 or kStrip - 1
+      const bool in0 = t0 >= 0 && t0 < kStrip, in1 = t0 + 1 >= 0 && t0 + 1 < kStrip;
+      const float* col = volT + ((size_t)y * g.nx + x) * g.nz;
+      for (int ps0 = 0; ps0 < npass; ps0 += kPassBatch) {
+        // issue all loads of up to kPassBatch passes before the first use
+        float vals[kPassBatch];
 #pragma unroll
-      for (int j = 0; j < ZT; ++j) {
-        int row;
-        float fv;
-        bool okv;
-        column_row(cv, cp.joff + j, row, fv, okv);
-        if (!okv || val[j] == 0.f) continue;
-        const float sw = cv.w * val[j];
-        const float b = sw * fv, a = sw - b;
-        const float a1 = a * cv.fu, b1 = b * cv.fu;
-        if (lds) {
-          float* p = patch + (row - v0) * fu + (cv.iu - u0);
-          atomicAdd(p, a - a1);
-          atomicAdd(p + 1, a1);
-          atomicAdd(p + fu, b - b1);
-          atomicAdd(p + fu + 1, b1);
-        } else {
-          float* p = out + (size_t)(row + kPadLo) * g.pitch + cv.iu + kPadLo;
-          atomicAdd(p, a - a1);
-          atomicAdd(p + 1, a1);
-          atomicAdd(p + g.pitch, b - b1);
-          atomicAdd(p + g.pitch + 1, b1);
+        for (int q = 0; q < kPassBatch; ++q) {
+          const int z = (ps0 + q) * 64 + lane;
+          vals[q] = (ps0 + q < npass && z < g.nz) ? col[z] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < kPassBatch; ++q) {
+          const int ps = ps0 + q;
+          const int z = ps * 64 + lane;
+          const float v = in_mask(g, x, y, z) ? vals[q] : 0.f;
+          if (v == 0.f) continue;
+          const ChunkBase cb = chunk_base(cc, c, base_z(g, ps * 64 + grp), g);
+          int row;
+          float fv;
+          bool okv;
+          chunk_row(cc, cb, jj, row, fv, okv);
+          if (!okv) continue;
+          const float sw = cc.w * v;
+          const float bb = sw * fv, aa = sw - bb;
+          const float a1 = aa * cc.fu, b1 = bb * cc.fu;
+          float* r0 = strip + (row + kPadLo) * kStripLd;
+          float* r1 = r0 + kStripLd;
+          if (in0) { atomicAdd(r0 + t0, aa - a1); atomicAdd(r1 + t0, bb - b1); }
+          if (in1) { atomicAdd(r0 + t0 + 1, a1); atomicAdd(r1 + t0 + 1, b1); }
         }
       }
     }
-    __syncthreads();
-    if (lds) {
-      for (int i = tid; i < fu * fvn; i += 256) {
-        const float v = patch[i];
-        if (v != 0.f) {
-          const int r = i / fu, c = i - r * fu;
-          atomicAdd(out + (size_t)(v0 + r + kPadLo) * g.pitch + u0 + c + kPadLo, v);
-        }
-      }
-    }
-    __syncthreads();
+  }
+  __syncthreads();
+  float* out = filt + (size_t)k * g.rows * g.pitch;
+  for (int i = tid; i < g.rows * kStrip; i += 256) {
+    const int r = i / kStrip, t = i - r * kStrip;
+    if (pu0 + t < g.pitch) out[(size_t)r * g.pitch + pu0 + t] = strip[r * kStripLd + t];
   }
 }
 
-hipError_t launch_project(const float* vol, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
+// vol [z][y][x] -> volT [y][x][z] through 32 x 32 LDS tiles (per y).
+__global__ __launch_bounds__(256) void transpose_zx_kernel(const float* __restrict__ vol, float* __restrict__ volT,
+                                                           int nx, int ny, int nz) {
+  __shared__ float t[32][33];
+  const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int z = z0 + r, x = x0 + tx;
+    t[r][tx] = (z < nz && x < nx) ? vol[((size_t)z * ny + y) * nx + x] : 0.f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int x = x0 + r, z = z0 + tx;
+    if (x < nx && z < nz) volT[((size_t)y * nx + x) * nz + z] = t[tx][r];
+  }
+}
+
+hipError_t launch_transpose_zx(const float* vol, float* volT, int nx, int ny, int nz, hipStream_t s) {
+  dim3 gd((nx + 31) / 32, (nz + 31) / 32, ny);
+  hipLaunchKernelGGL(transpose_zx_kernel, gd, dim3(256), 0, s, vol, volT, nx, ny, nz);
+  return hipGetLastError();
+}
+
+hipError_t launch_project(const float* volT, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
                           float* filt_pad, const monte_tuning& t, hipStream_t s) {
-  constexpr int ZT = 16;
-  int tile_x, tiles_x, tiles_y;
-  long nblocks;
-  monte_tuning tt = t;
-  tt.bp_tile_x = 16;
-  bp_tiling(grid, tt, ZT, tile_x, tiles_x, tiles_y, nblocks);
+  (void)t;
+  const int n_strips = (grid.pitch + kStrip - 1) / kStrip;
+  const long nblocks = (long)n_strips * nviews_chunk;
   if (nblocks <= 0) return hipSuccess;
   if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((project_kernel<ZT>), dim3((unsigned)nblocks), dim3(256), 0, s, vol, views, view0,
-                     nviews_chunk, grid, filt_pad, tile_x, tiles_x, tiles_y);
+  const size_t lds = sizeof(float) * (size_t)grid.rows * kStripLd;
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(project_kernel, dim3((unsigned)nblocks), dim3(256), lds, s, volT, views, view0, n_strips, grid,
+                     filt_pad);
   return hipGetLastError();
 }
 

@@ -144,7 +144,8 @@ struct monte_ctx {
   //   0 forward filtered stack (padded)   1 adjoint filtered stack (padded)
   //   2 MC descriptors                    3 FDK constants (taps + per-view coefficients)
   //   4..7 staging for host-pointer mode and tuning scratch
-  static constexpr int kSlots = 8;
+  //   8 z-fastest copy of the volume for the matched projector
+  static constexpr int kSlots = 9;
   void* ws[kSlots] = {};
   size_t ws_size[kSlots] = {};
 
@@ -185,8 +186,11 @@ hipError_t launch_fill_pads(float* buf, int n_views, int nu, int nv, int pitch, 
 hipError_t launch_fold_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, hipStream_t s);
 hipError_t launch_backproject(const float* filt_pad, const BpView* views, int view0, int nviews_chunk,
                               const BpGrid& grid, float* vol, int accumulate, const monte_tuning& t, hipStream_t s);
-hipError_t launch_project(const float* vol, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
+// volT: the volume with z fastest ([y][x][z], see launch_transpose_zx); every
+// cell of the padded views [view0, view0 + nviews_chunk) is written.
+hipError_t launch_project(const float* volT, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
                           float* filt_pad, const monte_tuning& t, hipStream_t s);
+hipError_t launch_transpose_zx(const float* vol, float* volT, int nx, int ny, int nz, hipStream_t s);
 hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* taps, const FilterParams& fp,
                                hipStream_t s);
 
