@@ -141,6 +141,7 @@ typedef struct monte_tuning {
   int32_t bp_zchunk;         /* voxels along z per backprojection thread */
   int32_t bp_tile_x, bp_tile_y; /* backprojection block tile (threads) */
   int32_t bp_view_chunk;     /* views per backprojection launch */
+  int32_t pj_zchunk;         /* projector voxels per lane along z: 2, 4, 8, or 0 = chosen from nz */
 } monte_tuning;
 
 /* ------------------------------------------------------------ context */

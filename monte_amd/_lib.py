@@ -76,7 +76,8 @@ class McJob(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
-        "mc_block", "mc_batch", "mc_blocks_per_cu", "bp_zchunk", "bp_tile_x", "bp_tile_y", "bp_view_chunk")]
+        "mc_block", "mc_batch", "mc_blocks_per_cu", "bp_zchunk", "bp_tile_x", "bp_tile_y", "bp_view_chunk",
+        "pj_zchunk")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

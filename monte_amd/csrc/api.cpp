@@ -389,6 +389,7 @@ monte_tuning default_tuning() {
   t.bp_tile_x = 16;
   t.bp_tile_y = 16;
   t.bp_view_chunk = 0;
+  t.pj_zchunk = 0;
   return t;
 }
 
@@ -479,6 +480,8 @@ int monte_set_tuning(monte_ctx* c, const monte_tuning* in) {
   if (t.bp_tile_x <= 0 || 256 % t.bp_tile_x) return fail(c, MONTE_ERR_INVALID, "bp_tile_x");
   t.bp_tile_y = 256 / t.bp_tile_x;
   if (t.bp_view_chunk < 0) return fail(c, MONTE_ERR_INVALID, "bp_view_chunk");
+  if (t.pj_zchunk != 0 && t.pj_zchunk != 2 && t.pj_zchunk != 4 && t.pj_zchunk != 8)
+    return fail(c, MONTE_ERR_INVALID, "pj_zchunk");
   c->tuning = t;
   return MONTE_OK;
 }

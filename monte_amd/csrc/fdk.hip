@@ -496,21 +496,32 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 }
 
 // ------------------------------------------- transposed backprojection
-// The matched projector, written as a gather so no global atomics are needed.
-// A block owns a strip of kStrip padded detector columns (all rows) of one
-// view, held in LDS.  It enumerates the voxel columns whose left tap can land
-// in the strip: for each slab of the volume across the view's rays (x-slabs
-// when |cos b| >= |sin b|, y-slabs otherwise) the projective map su(X,Y) is
-// inverted at the strip's two edges, giving the range of columns on that slab
-// (widened by one voxel each side; every column is then tested exactly).  A
-// wave takes one column at a time and its lanes walk z over a z-fastest copy
-// of the volume, 64 voxels per pass, with the backprojector's own per-voxel
-// arithmetic (same column core, same 16-aligned bases), so every weight equals
-// the backprojector's bit for bit.  Each voxel deposits its (up to four) taps
-// that fall inside the strip with LDS atomics; the strip is then stored once.
-constexpr int kStrip = 8;
-constexpr int kStripLd = kStrip + 1;  // padded LDS row: consecutive rows fall in different banks
-constexpr int kPassBatch = 8;         // 64-voxel passes whose loads are issued together
+// The matched projector, written as a gather so no atomics are needed.
+// One wave owns a strip of kStripW padded detector columns (all rows) of one
+// view, held in its own LDS.  It enumerates the voxel columns whose left tap
+// can land in the strip: for each slab of the volume across the view's rays
+// (x-slabs when |cos b| >= |sin b|, y-slabs otherwise) the projective map
+// su(X,Y) is inverted at the strip's two edges, giving a range of columns on
+// that slab (widened by one voxel each side).  The lanes evaluate the
+// candidate columns of a range in parallel with the backprojector's own
+// column arithmetic and keep those whose taps fall in the strip.  Each kept
+// column is then walked along z by the whole wave: lane l takes the C
+// consecutive voxels of chunk l, from a z-fastest copy of the volume, with the
+// backprojector's per-voxel arithmetic (same column core, same 16-aligned
+// bases), so every weight equals the backprojector's bit for bit.
+//
+// Accumulation is a plain read-add-write of LDS, without atomics: in one
+// instruction the active lanes' voxels are C apart along z, hence at least
+// C |dsv| rows apart on the detector, and when C |dsv| > 1.01 no two lanes
+// touch the same cell.  Columns with a smaller v-slope run in several phases
+// over interleaved lane subsets that satisfy the same bound.  The upper tap
+// row is updated by a second read-add-write after the first completes, and a
+// wave's LDS operations execute in program order, so updates of a cell by
+// neighbouring lanes, later voxels and later columns all see each other.
+// Strip columns 0 and kStripW + 1 are guards that absorb taps landing just
+// outside the strip; the row pitch is odd so strided rows spread over banks.
+constexpr int kStripW = 7;
+constexpr int kStripP = kStripW + 2;  // odd
 
 __device__ inline void slab_range(double s_lo, double s_hi, const BpView& c, double coord, bool x_slabs,
                                   double o0, double os, int n, int& a, int& b) {
@@ -537,17 +548,73 @@ __device__ inline void slab_range(double s_lo, double s_hi, const BpView& c, dou
   b = (int)fmin(ceil(hi) + 1.0, (double)(n - 1));
 }
 
-__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ volT,
-                                                      const BpView* __restrict__ views, int view0, int n_strips,
-                                                      BpGrid g, float* __restrict__ filt) {
-  extern __shared__ float strip[];  // [rows][kStripLd]
+__device__ inline int bcast_i(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ inline float bcast_f(float v, int src) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src)); }
+__device__ inline double bcast_d(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Keeps the compiler from moving LDS accesses across this point; the hardware
+// already executes one wave's LDS operations in order.
+__device__ inline void lds_order() { asm volatile("" ::: "memory"); }
+
+template <int C>
+__device__ inline void project_column(float* __restrict__ strip, const float* __restrict__ col, const BpView& c,
+                                      const ColumnCore& cc, int t0, int x, int y, const BpGrid& g, int lane) {
+  static_assert(kZBase % C == 0, "a chunk must lie inside one 16-aligned base group");
+  const float step = (float)C * fabsf(cc.dsv);
+  const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
+  const int nch = (g.nz + C - 1) / C;
+  for (int ch0 = 0; ch0 < nch; ch0 += 64) {
+    const int ch = ch0 + lane;
+    const int z0 = ch * C;
+    float v[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int z = z0 + j;
+      v[j] = (z < g.nz && in_mask(g, x, y, z)) ? col[z] : 0.f;
+    }
+    const int zb = (z0 / kZBase) * kZBase;
+    const ChunkBase cb = chunk_base(cc, c, base_z(g, zb), g);
+    const int joff = z0 - zb;
+    for (int ph = 0; ph < phases; ++ph) {
+      const bool act = (lane % phases) == ph;
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        int row;
+        float fv;
+        bool okv;
+        chunk_row(cc, cb, joff + j, row, fv, okv);
+        const bool live = act && okv && v[j] != 0.f;
+        const float sw = cc.w * v[j];
+        const float bb = sw * fv, aa = sw - bb;
+        const float a1 = aa * cc.fu, b1 = bb * cc.fu;
+        float* r0 = strip + (row + kPadLo) * kStripP + (t0 + 1);
+        if (live) { r0[0] += aa - a1; r0[1] += a1; }
+        lds_order();
+        float* r1 = r0 + kStripP;
+        if (live) { r1[0] += bb - b1; r1[1] += b1; }
+        lds_order();
+      }
+    }
+  }
+}
+
+template <int C>
+__global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ volT,
+                                                     const BpView* __restrict__ views, int view0, int n_strips,
+                                                     BpGrid g, float* __restrict__ filt) {
+  extern __shared__ float strip[];  // [rows][kStripP]; column q holds padded detector column pu0 + q - 1
   const int k = view0 + blockIdx.x / n_strips;
   const int st = blockIdx.x - (k - view0) * n_strips;
-  const int pu0 = st * kStrip;       // first padded column of the strip
+  const int pu0 = st * kStripW;      // first padded column of the strip
   const int u_lo = pu0 - kPadLo;     // first detector column index owned
-  const int u_hi = u_lo + kStrip - 1;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int i = tid; i < g.rows * kStripLd; i += 256) strip[i] = 0.f;
+  const int u_hi = u_lo + kStripW - 1;
+  const int lane = threadIdx.x;
+  for (int i = lane; i < g.rows * kStripP; i += 64) strip[i] = 0.f;
   __syncthreads();
 
   const BpView c = views[k];  // held in registers: no constant reloads inside the loops
@@ -555,9 +622,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
   // columns whose left tap iu satisfies u_lo - 1 <= iu <= u_hi, i.e. su in [u_lo - 1, u_hi + 1)
   const double s_lo = (double)(u_lo - 1) - 0.5, s_hi = (double)(u_hi + 1) + 0.5;
   const int n_slabs = x_slabs ? (g.x1 - g.x0) : g.ny;
-  const int npass = (g.nz + 63) / 64;
-  const int grp = lane & ~(kZBase - 1), jj = lane & (kZBase - 1);
-  for (int sl = wave; sl < n_slabs; sl += 4) {
+  for (int sl = 0; sl < n_slabs; ++sl) {
     int a, b;
     int xs = 0, ys = 0;
     if (x_slabs) {
@@ -569,50 +634,36 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
       a = max(a, g.x0);
       b = min(b, g.x1 - 1);
     }
-    for (int o = a; o <= b; ++o) {
+    for (int o0 = a; o0 <= b; o0 += 64) {
+      const int o = min(o0 + lane, b);
       const int x = x_slabs ? xs : o, y = x_slabs ? o : ys;
       const double X = voxel_x(g, x), Y = voxel_y(g, y);
-      const ColumnCore cc = column_core(c, X, Y, (float)X, (float)Y, g);
-      if (!cc.ok || cc.iu < u_lo - 1 || cc.iu > u_hi) continue;  // wave-uniform
-      const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap, may be -1 or kStrip - 1
-      const bool in0 = t0 >= 0 && t0 < kStrip, in1 = t0 + 1 >= 0 && t0 + 1 < kStrip;
-      const float* col = volT + ((size_t)y * g.nx + x) * g.nz;
-      for (int ps0 = 0; ps0 < npass; ps0 += kPassBatch) {
-        // issue all loads of up to kPassBatch passes before the first use
-        float vals[kPassBatch];
-#pragma unroll
-        for (int q = 0; q < kPassBatch; ++q) {
-          const int z = (ps0 + q) * 64 + lane;
-          vals[q] = (ps0 + q < npass && z < g.nz) ? col[z] : 0.f;
-        }
-#pragma unroll
-        for (int q = 0; q < kPassBatch; ++q) {
-          const int ps = ps0 + q;
-          const int z = ps * 64 + lane;
-          const float v = in_mask(g, x, y, z) ? vals[q] : 0.f;
-          if (v == 0.f) continue;
-          const ChunkBase cb = chunk_base(cc, c, base_z(g, ps * 64 + grp), g);
-          int row;
-          float fv;
-          bool okv;
-          chunk_row(cc, cb, jj, row, fv, okv);
-          if (!okv) continue;
-          const float sw = cc.w * v;
-          const float bb = sw * fv, aa = sw - bb;
-          const float a1 = aa * cc.fu, b1 = bb * cc.fu;
-          float* r0 = strip + (row + kPadLo) * kStripLd;
-          float* r1 = r0 + kStripLd;
-          if (in0) { atomicAdd(r0 + t0, aa - a1); atomicAdd(r1 + t0, bb - b1); }
-          if (in1) { atomicAdd(r0 + t0 + 1, a1); atomicAdd(r1 + t0 + 1, b1); }
-        }
+      const ColumnCore mine = column_core(c, X, Y, (float)X, (float)Y, g);
+      const bool keep = o0 + lane <= b && mine.ok && mine.iu >= u_lo - 1 && mine.iu <= u_hi;
+      uint64_t m = __ballot(keep);
+      while (m) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1;
+        ColumnCore cc;
+        cc.iu = bcast_i(mine.iu, src);
+        cc.fu = bcast_f(mine.fu, src);
+        cc.w = bcast_f(mine.w, src);
+        cc.ok = true;
+        cc.kv = bcast_d(mine.kv, src);
+        cc.dsv = bcast_f(mine.dsv, src);
+        const int oc = bcast_i(o, src);
+        const int cx = x_slabs ? xs : oc, cy = x_slabs ? oc : ys;
+        const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: -1 .. kStripW - 1
+        const float* col = volT + ((size_t)cy * g.nx + cx) * g.nz;
+        project_column<C>(strip, col, c, cc, t0, cx, cy, g, lane);
       }
     }
   }
   __syncthreads();
   float* out = filt + (size_t)k * g.rows * g.pitch;
-  for (int i = tid; i < g.rows * kStrip; i += 256) {
-    const int r = i / kStrip, t = i - r * kStrip;
-    if (pu0 + t < g.pitch) out[(size_t)r * g.pitch + pu0 + t] = strip[r * kStripLd + t];
+  for (int i = lane; i < g.rows * kStripW; i += 64) {
+    const int r = i / kStripW, t = i - r * kStripW;
+    if (pu0 + t < g.pitch) out[(size_t)r * g.pitch + pu0 + t] = strip[r * kStripP + t + 1];
   }
 }
 
@@ -641,15 +692,28 @@ hipError_t launch_transpose_zx(const float* vol, float* volT, int nx, int ny, in
 
 hipError_t launch_project(const float* volT, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
                           float* filt_pad, const monte_tuning& t, hipStream_t s) {
-  (void)t;
-  const int n_strips = (grid.pitch + kStrip - 1) / kStrip;
+  const int n_strips = (grid.pitch + kStripW - 1) / kStripW;
   const long nblocks = (long)n_strips * nviews_chunk;
   if (nblocks <= 0) return hipSuccess;
   if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
-  const size_t lds = sizeof(float) * (size_t)grid.rows * kStripLd;
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(project_kernel, dim3((unsigned)nblocks), dim3(256), lds, s, volT, views, view0, n_strips, grid,
-                     filt_pad);
+  const size_t lds = sizeof(float) * (size_t)grid.rows * kStripP;
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  // voxels per lane: fill the wave's 64 lanes best over nz, favouring short chunks
+  int best = t.pj_zchunk;
+  long best_cost = -1;
+  for (int cz : {2, 4, 8}) {
+    if (t.pj_zchunk) break;
+    const long passes = ((grid.nz + cz - 1) / cz + 63) / 64;
+    const long cost = passes * (cz * 30 + 20);
+    if (best_cost < 0 || cost < best_cost) { best = cz; best_cost = cost; }
+  }
+  const dim3 gd((unsigned)nblocks), bd(64);
+  switch (best) {
+    case 2: hipLaunchKernelGGL(project_kernel<2>, gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad); break;
+    case 4: hipLaunchKernelGGL(project_kernel<4>, gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad); break;
+    case 8: hipLaunchKernelGGL(project_kernel<8>, gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -1,7 +1,9 @@
 // Host-side geometry: per-view backprojection coefficients and filter
-// constants for the two presets, computed in FP64 and handed to the kernels in
-// FP32.  One set of kernels serves both presets; the presets differ only in
-// these numbers (see BpView in internal.h).
+// constants for the two presets, computed in FP64.  The projective map of each
+// view stays FP64 (the kernels form each column's U, 1/U and sample
+// coordinates from it in FP64); the distance-weight terms and the filter
+// constants are handed over in FP32.  One set of kernels serves both presets;
+// the presets differ only in these numbers (see BpView in internal.h).
 #include <cmath>
 
 #include "internal.h"

@@ -48,6 +48,8 @@ struct Stream {
   int left;
 };
 
+__device__ inline float u01(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-08f; }  // 2^-24
+
 __device__ inline float next_u01(Stream& s, uint32_t job, uint32_t k0, uint32_t k1) {
   if (s.left == 0) {
     uint32_t c0 = s.ctr, c1 = s.photon, c2 = s.view, c3 = job;
@@ -59,7 +61,7 @@ __device__ inline float next_u01(Stream& s, uint32_t job, uint32_t k0, uint32_t 
   const uint32_t x = s.b0;
   s.b0 = s.b1; s.b1 = s.b2; s.b2 = s.b3;
   s.left -= 1;
-  return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-08f;  // 2^-24
+  return u01(x);
 }
 
 // --------------------------------------------------------------- physics
@@ -160,6 +162,7 @@ struct Photon {
   float t, t1;         // flight parameter and box-exit parameter
   float mumax;         // Woodcock majorant of the current table row
   float inv_mumax;
+  float mu0, mu1, mu2, mu3;  // mu of labels 0..3 at the current table row
   int eb;              // table row
   int lab;             // label at the pending collision
   int nint;            // real interactions so far
@@ -198,6 +201,10 @@ __device__ inline bool start_flight(Photon& p, const McJob& J) {
   if (!ray_box(p.px, p.py, p.pz, p.dx, p.dy, p.dz, J, t0, t1) || !(t1 > 0.f)) return false;
   const float mm = J.tables->mumax[p.eb];
   if (!(mm > 0.f)) return false;
+  p.mu0 = J.tables->m[0][p.eb].x;
+  p.mu1 = J.tables->m[1][p.eb].x;
+  p.mu2 = J.tables->m[2][p.eb].x;
+  p.mu3 = J.tables->m[3][p.eb].x;
   if (t0 > 0.f) {
     p.px = fmaf(t0, p.dx, p.px); p.py = fmaf(t0, p.dy, p.py); p.pz = fmaf(t0, p.dz, p.pz);
     t1 -= t0;
@@ -207,6 +214,23 @@ __device__ inline bool start_flight(Photon& p, const McJob& J) {
   p.mumax = mm;
   p.inv_mumax = 1.0f / mm;
   return true;
+}
+
+// Attenuation at a label: labels 0..3 from registers, others from the table.
+__device__ inline float mu_at(const Photon& p, int lab, const McJob& J) {
+  const bool odd = (lab & 1) != 0;
+  const float lo = odd ? p.mu1 : p.mu0, hi = odd ? p.mu3 : p.mu2;
+  float mu = (lab & 2) ? hi : lo;
+  if (lab > 3) mu = J.tables->m[lab][p.eb].x;
+  return mu;
+}
+
+__device__ inline int label_at(const Photon& p, float t, const McJob& J, float& qx, float& qy, float& qz) {
+  qx = fmaf(t, p.dx, p.px); qy = fmaf(t, p.dy, p.py); qz = fmaf(t, p.dz, p.pz);
+  const int ix = min(max((int)floorf((qx - J.bx0) * J.inv_vx), 0), J.nx - 1);
+  const int iy = min(max((int)floorf((qy - J.by0) * J.inv_vy), 0), J.ny - 1);
+  const int iz = min(max((int)floorf((qz - J.bz0) * J.inv_vz), 0), J.nz - 1);
+  return J.labels[((size_t)iz * J.ny + iy) * J.nx + ix];
 }
 
 // Wave-uniform statistics (scalar registers), flushed per job.
@@ -349,6 +373,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       }
       if (state == kCollide) {
         const float4 m = Jp->tables->m[p.lab][p.eb];
+        rs.left = 0;  // interactions start on a fresh block
         const float u3 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
         if (u3 <= m.y) {
           is_photo = true;
@@ -390,33 +415,48 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       count_tally(cnt, tallied, code);
     }
 
-    // 4) one Woodcock step for every flying lane
-    bool stepped = false, tallied = false;
+    // 4) two Woodcock steps for every flying lane from one Philox block
+    //    (distance, accept, distance, accept); both label loads are issued
+    //    before either is used, and a flight ending on the first step leaves
+    //    the block's other two uniforms unused.
+    bool stepped = false, second = false, tallied = false;
     int code = 0;
     if (state == kFly) {
       stepped = true;
-      const float u = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
-      p.t = fmaf(-__logf(u), p.inv_mumax, p.t);
-      if (p.t >= p.t1) {
+      uint32_t c0 = rs.ctr, c1 = rs.photon, c2 = rs.view, c3 = Jp->job_id;
+      philox10(c0, c1, c2, c3, Jp->key0, Jp->key1);
+      rs.ctr += 1;
+      rs.left = 0;
+      const float ta = fmaf(-__logf(u01(c0)), p.inv_mumax, p.t);
+      const float tb = fmaf(-__logf(u01(c2)), p.inv_mumax, ta);
+      float ax, ay, az, bx, by, bz;
+      const int la = label_at(p, ta, *Jp, ax, ay, az);
+      const int lb = label_at(p, tb, *Jp, bx, by, bz);
+      const float mua = mu_at(p, la, *Jp), mub = mu_at(p, lb, *Jp);
+      if (ta >= p.t1) {
         code = tally(p, *Jp);
         tallied = true;
         state = kDead;
+      } else if (u01(c1) * p.mumax < mua) {  // real collision: park until the wave runs interactions
+        p.px = ax; p.py = ay; p.pz = az;
+        p.lab = la;
+        state = kCollide;
       } else {
-        const float qx = fmaf(p.t, p.dx, p.px), qy = fmaf(p.t, p.dy, p.py), qz = fmaf(p.t, p.dz, p.pz);
-        const int ix = min(max((int)floorf((qx - Jp->bx0) * Jp->inv_vx), 0), Jp->nx - 1);
-        const int iy = min(max((int)floorf((qy - Jp->by0) * Jp->inv_vy), 0), Jp->ny - 1);
-        const int iz = min(max((int)floorf((qz - Jp->bz0) * Jp->inv_vz), 0), Jp->nz - 1);
-        const int lab = Jp->labels[((size_t)iz * Jp->ny + iy) * Jp->nx + ix];
-        const float mu = Jp->tables->m[lab][p.eb].x;
-        const float u2 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
-        if (u2 * p.mumax < mu) {  // real collision: park until the wave runs interactions
-          p.px = qx; p.py = qy; p.pz = qz;
-          p.lab = lab;
+        second = true;
+        if (tb >= p.t1) {
+          code = tally(p, *Jp);
+          tallied = true;
+          state = kDead;
+        } else if (u01(c3) * p.mumax < mub) {
+          p.px = bx; p.py = by; p.pz = bz;
+          p.lab = lb;
           state = kCollide;
+        } else {
+          p.t = tb;
         }
       }
     }
-    cnt.steps += popc(__ballot(stepped));
+    cnt.steps += popc(__ballot(stepped)) + popc(__ballot(second));
     count_tally(cnt, tallied, code);
   }
   if (job >= 0) flush_counters(cnt, counters + job);

@@ -83,7 +83,15 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 double orc_u01(uint32_t x) { return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0); }
 
-/* per-photon uniform stream: counter (block, photon, view, job), key (seed) */
+/* Per-photon uniform stream: block b of photon n in view k of job j is
+ * Philox4x32-10 at counter (b, n, k, j) with key (seed lo, seed hi).  Draw
+ * order, shared with the product kernel:
+ *   - the spectrum draw (if any) is the first uniform of block 0;
+ *   - every Woodcock flight takes whole blocks, two steps per block as
+ *     (distance, accept, distance, accept); a flight ending on the first
+ *     step of a block leaves the other two unused;
+ *   - every interaction starts on a fresh block and draws in sequence; the
+ *     rest of its last block is unused (the next flight takes a new block). */
 typedef struct {
   uint32_t ctr[4];
   uint32_t key[2];
@@ -105,6 +113,16 @@ static double orc_next(orc_stream *s) {
     s->idx = 0;
   }
   return orc_u01(s->buf[s->idx++]);
+}
+
+/* Whole next block as four uniforms; sequential draws after it start on a
+ * fresh block. */
+static void orc_block(orc_stream *s, double u[4]) {
+  uint32_t b[4];
+  orc_philox4x32_10(s->ctr, s->key, b);
+  s->ctr[0] += 1;
+  s->idx = 4;
+  for (int i = 0; i < 4; ++i) u[i] = orc_u01(b[i]);
 }
 
 /* ================================================================ tables */
@@ -723,52 +741,56 @@ static void mc_items(void *vctx, long i0, long i1) {
           }
           double t = 0;
           const double mumax = mt->mumax[eb];
-          for (;;) {
-            t += -log(orc_next(&rs)) / mumax;
-            st.steps++;
-            if (t >= t1) break;
-            double P[3] = {pos[0] + t * dir[0], pos[1] + t * dir[1], pos[2] + t * dir[2]};
-            int ix = (int)floor((P[0] - b0[0]) / ph->vx), iy = (int)floor((P[1] - b0[1]) / ph->vy),
-                iz = (int)floor((P[2] - b0[2]) / ph->vz);
-            ix = ix < 0 ? 0 : (ix >= ph->nx ? ph->nx - 1 : ix);
-            iy = iy < 0 ? 0 : (iy >= ph->ny ? ph->ny - 1 : iy);
-            iz = iz < 0 ? 0 : (iz >= ph->nz ? ph->nz - 1 : iz);
-            int lab = ph->labels[((size_t)iz * ph->ny + iy) * ph->nx + ix];
-            double mu = mt->mu[lab][eb];
-            double u2 = orc_next(&rs);
-            if (u2 * mumax < mu) {
-              real = 1;
-              pos[0] = P[0]; pos[1] = P[1]; pos[2] = P[2];
-              /* interaction at this label */
-              if (nint >= cfg->max_interactions) {
-                st.capped++;
-                goto done;
-              }
-              double u3 = orc_next(&rs);
-              st.interactions++;
-              if (u3 <= mt->fa[lab][eb]) {
-                st.photo++;
-                goto done;
-              } else if (u3 <= mt->fac[lab][eb]) {
-                st.coherent++;
-                if (cfg->coherent_mode == ORC_COH_RAYLEIGH) {
-                  double cth = orc_thomson_mu(orc_next(&rs));
+          /* two Woodcock steps per block: (distance, accept, distance, accept) */
+          for (int ended = 0; !ended;) {
+            double u[4];
+            orc_block(&rs, u);
+            for (int h = 0; h < 2 && !ended; ++h) {
+              t += -log(u[2 * h]) / mumax;
+              st.steps++;
+              if (t >= t1) { ended = 1; break; }
+              double P[3] = {pos[0] + t * dir[0], pos[1] + t * dir[1], pos[2] + t * dir[2]};
+              int ix = (int)floor((P[0] - b0[0]) / ph->vx), iy = (int)floor((P[1] - b0[1]) / ph->vy),
+                  iz = (int)floor((P[2] - b0[2]) / ph->vz);
+              ix = ix < 0 ? 0 : (ix >= ph->nx ? ph->nx - 1 : ix);
+              iy = iy < 0 ? 0 : (iy >= ph->ny ? ph->ny - 1 : iy);
+              iz = iz < 0 ? 0 : (iz >= ph->nz ? ph->nz - 1 : iz);
+              int lab = ph->labels[((size_t)iz * ph->ny + iy) * ph->nx + ix];
+              double mu = mt->mu[lab][eb];
+              if (u[2 * h + 1] * mumax < mu) {
+                ended = 1;
+                real = 1;
+                pos[0] = P[0]; pos[1] = P[1]; pos[2] = P[2];
+                /* interaction at this label */
+                if (nint >= cfg->max_interactions) {
+                  st.capped++;
+                  goto done;
+                }
+                double u3 = orc_next(&rs); /* fresh block: orc_block left none */
+                st.interactions++;
+                if (u3 <= mt->fa[lab][eb]) {
+                  st.photo++;
+                  goto done;
+                } else if (u3 <= mt->fac[lab][eb]) {
+                  st.coherent++;
+                  if (cfg->coherent_mode == ORC_COH_RAYLEIGH) {
+                    double cth = orc_thomson_mu(orc_next(&rs));
+                    double phi = 2.0 * M_PI * orc_next(&rs);
+                    rotate_dir(dir, cth, phi);
+                  }
+                } else {
+                  st.compton++;
+                  double ro = kahn_sample(E, stream_next, &rs);
+                  double lam = 511.0 / E, lamd = ro * lam;
+                  double cth = 1.0 - (lamd - lam);
+                  if (cth < -1) cth = -1;
+                  E = 511.0 / lamd;
+                  eb = ebin_of(E);
                   double phi = 2.0 * M_PI * orc_next(&rs);
                   rotate_dir(dir, cth, phi);
                 }
-              } else {
-                st.compton++;
-                double ro = kahn_sample(E, stream_next, &rs);
-                double lam = 511.0 / E, lamd = ro * lam;
-                double cth = 1.0 - (lamd - lam);
-                if (cth < -1) cth = -1;
-                E = 511.0 / lamd;
-                eb = ebin_of(E);
-                double phi = 2.0 * M_PI * orc_next(&rs);
-                rotate_dir(dir, cth, phi);
+                nint++;
               }
-              nint++;
-              break;
             }
           }
         }

@@ -32,10 +32,15 @@ def _geoms():
     compat = G.bp3d20_65(n_views=24).replace(nx=64, ny=64, nz=64, roi_x0=20, roi_x1=44, vx=0.4, vy=0.4, vz=0.4,
                                              mask_r=-1.0)
     compat_mask = compat.replace(mask_r=25.0)
-    return {"std": std, "shepp": shepp, "bp3d20": compat, "bp3d20_mask": compat_mask}
+    # thin slices under a coarse detector: the v-slope per voxel is 0.4 and 0.2
+    # rows, so the projector interleaves 2 and 3 lane phases per column
+    thin = std.replace(nz=60, vz=0.02, n_views=12)
+    thinner = std.replace(nz=90, vz=0.01, n_views=8)
+    return {"std": std, "shepp": shepp, "bp3d20": compat, "bp3d20_mask": compat_mask, "thin": thin,
+            "thinner": thinner}
 
 
-@pytest.mark.parametrize("name", ["std", "shepp", "bp3d20", "bp3d20_mask"])
+@pytest.mark.parametrize("name", ["std", "shepp", "bp3d20", "bp3d20_mask", "thin", "thinner"])
 def test_dot_product_identity(ctx, name):
     from monte_amd.fdk import (fdk, fdk_adjoint, fdk_backproject, fdk_filter, fdk_filter_adjoint,
                                fdk_project)
@@ -60,10 +65,11 @@ def test_dot_product_identity(ctx, name):
     assert abs(f1 - f2) <= 1e-5 * abs(f1)
 
 
-def test_adjoint_matches_oracle(ctx, oracle):
+@pytest.mark.parametrize("name", ["std", "thinner"])
+def test_adjoint_matches_oracle(ctx, oracle, name):
     from monte_amd.fdk import fdk_adjoint
 
-    g = _geoms()["std"]
+    g = _geoms()[name]
     rng = np.random.default_rng(5)
     y = rng.standard_normal(g.vol_shape).astype(np.float32)
     got = fdk_adjoint(_t(y), g, ctx).cpu().numpy()
@@ -127,3 +133,29 @@ def test_autograd_batch_and_graph_of_ops(ctx):
     loss.backward()
     expect = 2.0 * fdk_adjoint((vol - target).detach().contiguous(), geom, ctx)
     assert torch.allclose(x.grad, expect, rtol=1e-6, atol=1e-6 * expect.abs().max().item())
+
+
+@pytest.mark.parametrize("name", ["std", "thin", "bp3d20_mask"])
+def test_projector_agrees_across_lane_chunks(ctx, name):
+    """Every voxels-per-lane variant of the projector is the same transpose:
+    each passes the dot-product identity, and they agree with each other up
+    to FP32 summation order."""
+    from monte_amd.fdk import fdk_backproject, fdk_project
+
+    g = _geoms()[name]
+    rng = np.random.default_rng(11)
+    y = rng.standard_normal(g.vol_shape).astype(np.float32)
+    f = rng.standard_normal(g.filt_shape).astype(np.float32)
+    bf = _dot(fdk_backproject(_t(f), g, ctx).cpu().numpy(), y)
+    saved = ctx.tuning
+    outs = {}
+    try:
+        for cz in (2, 4, 8):
+            ctx.set_tuning(pj_zchunk=cz)
+            outs[cz] = fdk_project(_t(y), g, ctx).cpu().numpy()
+            pf = _dot(f, outs[cz])
+            assert abs(bf - pf) <= 1e-5 * abs(bf), (cz, bf, pf)
+    finally:
+        ctx.set_tuning(**saved)
+    for cz in (4, 8):
+        assert rel_max(outs[cz], outs[2]) < 1e-6, cz
