@@ -181,10 +181,14 @@ int run_project_padded(monte_ctx* c, const monte_geom& g, const float* vol, floa
   float* volT = (float*)workspace(c, kSlotVolT, vol_elems(g) * sizeof(float));
   if (!volT) return fail(c, MONTE_ERR_NOMEM, "transposed volume allocation failed");
   MONTE_HIP(c, launch_transpose_zx(vol, volT, g.nx, g.ny, g.nz, c->stream));
+  std::vector<BpView> host_views;
+  compute_bp_views(g, host_views);
   const int chunk = c->tuning.bp_view_chunk > 0 ? c->tuning.bp_view_chunk : g.n_views;
   for (int v0 = 0; v0 < g.n_views; v0 += chunk) {
     const int n = std::min(chunk, g.n_views - v0);
-    MONTE_HIP(c, launch_project(volT, views, v0, n, grid, pad, c->tuning, c->stream));
+    int lds_r0, lds_rows;
+    projector_rows(host_views, v0, n, grid, lds_r0, lds_rows);
+    MONTE_HIP(c, launch_project(volT, views, v0, n, grid, pad, lds_r0, lds_rows, c->tuning, c->stream));
   }
   if (g.preset == MONTE_PRESET_BP3D20)
     MONTE_HIP(c, launch_fold_pads(pad, g.n_views, g.nu, g.nv, grid.pitch, grid.rows, c->stream));

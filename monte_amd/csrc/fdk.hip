@@ -561,101 +561,218 @@ __device__ inline double bcast_d(double v, int src) {
 // already executes one wave's LDS operations in order.
 __device__ inline void lds_order() { asm volatile("" ::: "memory"); }
 
+// Passes (64 chunks of C voxels) whose volume values are fetched one column
+// ahead; columns longer than kPrefetchZ / C / 64 passes load the rest in place.
+constexpr int kPrefetchZ = 512;
+// LDS floats after the strip that absorb the updates of lanes with nothing to
+// add (outside the v window or past the column end), so the updates need no
+// branches: two per lane, one row pitch and one cell of overlap.
+constexpr int kTrash = 2 * 64 + kStripP + 2;
+
 template <int C>
-__device__ inline void project_column(float* __restrict__ strip, const float* __restrict__ col, const BpView& c,
-                                      const ColumnCore& cc, int t0, int x, int y, const BpGrid& g, int lane) {
-  static_assert(kZBase % C == 0, "a chunk must lie inside one 16-aligned base group");
-  const float step = (float)C * fabsf(cc.dsv);
-  const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
-  const int nch = (g.nz + C - 1) / C;
-  for (int ch0 = 0; ch0 < nch; ch0 += 64) {
-    const int ch = ch0 + lane;
-    const int z0 = ch * C;
-    float v[C];
+struct ColumnVals {
+  static constexpr int NP = (kPrefetchZ / C + 63) / 64;
+  float v[NP][C];
+};
+
+template <bool MASK>
+__device__ inline bool in_mask_t(const BpGrid& g, int x, int y, int z) {
+  if (!MASK) return true;
+  const int dz = z - g.mcz, dy = y - g.mcy, dx = x - g.mcx;
+  return (float)(dz * dz + dy * dy + dx * dx) <= g.mask_r2;
+}
+
+// Volume values of chunk (p * 64 + lane) of a column, zero past the end and
+// outside the mask; the load itself is clamped into the column, so it is
+// unconditional.
+template <int C, bool MASK>
+__device__ inline void load_chunk(float (&v)[C], const float* __restrict__ col, int p, int x, int y,
+                                  const BpGrid& g, int lane) {
 #pragma unroll
-    for (int j = 0; j < C; ++j) {
-      const int z = z0 + j;
-      v[j] = (z < g.nz && in_mask(g, x, y, z)) ? col[z] : 0.f;
-    }
-    const int zb = (z0 / kZBase) * kZBase;
-    const ChunkBase cb = chunk_base(cc, c, base_z(g, zb), g);
-    const int joff = z0 - zb;
-    for (int ph = 0; ph < phases; ++ph) {
-      const bool act = (lane % phases) == ph;
-#pragma unroll
-      for (int j = 0; j < C; ++j) {
-        int row;
-        float fv;
-        bool okv;
-        chunk_row(cc, cb, joff + j, row, fv, okv);
-        const bool live = act && okv && v[j] != 0.f;
-        const float sw = cc.w * v[j];
-        const float bb = sw * fv, aa = sw - bb;
-        const float a1 = aa * cc.fu, b1 = bb * cc.fu;
-        float* r0 = strip + (row + kPadLo) * kStripP + (t0 + 1);
-        if (live) { r0[0] += aa - a1; r0[1] += a1; }
-        lds_order();
-        float* r1 = r0 + kStripP;
-        if (live) { r1[0] += bb - b1; r1[1] += b1; }
-        lds_order();
-      }
-    }
+  for (int j = 0; j < C; ++j) {
+    const int z = (p * 64 + lane) * C + j;
+    const float val = col[min(z, g.nz - 1)];
+    v[j] = (z < g.nz && in_mask_t<MASK>(g, x, y, z)) ? val : 0.f;
   }
 }
 
+template <int C, bool MASK>
+__device__ inline void load_column(ColumnVals<C>& cv, const float* __restrict__ col, int x, int y, const BpGrid& g,
+                                   int lane) {
+#pragma unroll
+  for (int p = 0; p < ColumnVals<C>::NP; ++p) load_chunk<C, MASK>(cv.v[p], col, p, x, y, g, lane);
+}
+
+// The C voxel updates of one lane; lanes with `act` false (or nothing to add)
+// update the trash floats instead.
 template <int C>
+__device__ inline void pass_updates(float* __restrict__ strip, const float (&v)[C], const ColumnCore& cc,
+                                    const ChunkBase& cb, int joff, int z0, bool act, int cell0, int trash,
+                                    const BpGrid& g) {
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    int row;
+    float fv;
+    bool okv;
+    chunk_row(cc, cb, joff + j, row, fv, okv);
+    const bool live = act && okv && z0 + j < g.nz;
+    const float sw = cc.w * v[j];
+    const float bb = sw * fv, aa = sw - bb;
+    const float a1 = aa * cc.fu, b1 = bb * cc.fu;
+    const int i0 = live ? cell0 + row * kStripP : trash;
+    strip[i0] += aa - a1;
+    strip[i0 + 1] += a1;
+    lds_order();
+    strip[i0 + kStripP] += bb - b1;
+    strip[i0 + kStripP + 1] += b1;
+    lds_order();
+  }
+}
+
+// One pass of a column: lane l takes the C voxels of chunk ch0 + l.  `cell0`
+// is the LDS index of (row 0, left tap); `trash` the lane's trash floats.
+template <int C>
+__device__ inline void project_pass(float* __restrict__ strip, const float (&v)[C], const BpView& c,
+                                    const ColumnCore& cc, int cell0, int trash, int ch0, int phases,
+                                    const BpGrid& g, int lane) {
+  const int z0 = (ch0 + lane) * C;
+  const int zb = z0 & ~(kZBase - 1);
+  const ChunkBase cb = chunk_base(cc, c, base_z(g, zb), g);
+  const int joff = z0 - zb;
+  if (phases == 1) {
+    pass_updates<C>(strip, v, cc, cb, joff, z0, true, cell0, trash, g);
+  } else {
+    const int ml = lane % phases;
+    for (int ph = 0; ph < phases; ++ph) pass_updates<C>(strip, v, cc, cb, joff, z0, ml == ph, cell0, trash, g);
+  }
+}
+
+// Up to 64 kept voxel columns of one strip, lane i holding the i-th.
+struct ColumnBatch {
+  int n = 0;
+  int iu = 0, x = 0, y = 0;
+  float fu = 0.f, w = 0.f, dsv = 0.f;
+  double kv = 0.0;
+};
+
+__device__ inline int permute_i(int dst, int v) { return __builtin_amdgcn_ds_permute(dst << 2, v); }
+__device__ inline float permute_f(int dst, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_permute(dst << 2, __float_as_int(v)));
+}
+
+// Walks the strip's slab ranges and appends the kept columns of each range to
+// the batch (each kept lane pushes its column to lane n + its rank; the other
+// lanes push to the lanes past the new end), until the batch is full or the
+// slabs are exhausted.  Returns false when nothing is left.
+struct Enumerator {
+  int sl = 0, o0 = 0, a = 0, b = -1, xs = 0, ys = 0;
+
+  __device__ bool fill(ColumnBatch& cb, const BpView& c, const BpGrid& g, bool x_slabs, int n_slabs, double s_lo,
+                       double s_hi, int u_lo, int u_hi, int lane) {
+    cb.n = 0;
+    for (;;) {
+      if (o0 > b) {
+        if (sl >= n_slabs) return cb.n > 0;
+        if (x_slabs) {
+          xs = g.x0 + sl;
+          slab_range(s_lo, s_hi, c, voxel_x(g, xs), true, g.gy0, g.gsy, g.ny, a, b);
+        } else {
+          ys = sl;
+          slab_range(s_lo, s_hi, c, voxel_y(g, ys), false, g.gx0, g.gsx, g.nx, a, b);
+          a = max(a, g.x0);
+          b = min(b, g.x1 - 1);
+        }
+        ++sl;
+        o0 = a;
+        continue;
+      }
+      const int o = min(o0 + lane, b);
+      const int x = x_slabs ? xs : o, y = x_slabs ? o : ys;
+      const double X = voxel_x(g, x), Y = voxel_y(g, y);
+      const ColumnCore mine = column_core(c, X, Y, (float)X, (float)Y, g);
+      const bool keep = o0 + lane <= b && mine.ok && mine.iu >= u_lo - 1 && mine.iu <= u_hi;
+      const uint64_t m = __ballot(keep);
+      const int cnt = (int)__popcll(m);
+      if (cb.n + cnt > 64) return true;  // full: this range is taken again by the next fill
+      o0 += 64;
+      if (!cnt) continue;
+      const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      const int dst = keep ? cb.n + (int)rk : (cb.n + cnt + (lane - (int)rk)) & 63;
+      const bool fresh = lane >= cb.n && lane < cb.n + cnt;
+      const long long kvb = __double_as_longlong(mine.kv);
+      const int kv_lo = permute_i(dst, (int)(kvb & 0xffffffffLL)), kv_hi = permute_i(dst, (int)(kvb >> 32));
+      const int piu = permute_i(dst, mine.iu), px = permute_i(dst, x), py = permute_i(dst, y);
+      const float pfu = permute_f(dst, mine.fu), pw = permute_f(dst, mine.w), pdsv = permute_f(dst, mine.dsv);
+      if (fresh) {
+        cb.iu = piu; cb.x = px; cb.y = py;
+        cb.fu = pfu; cb.w = pw; cb.dsv = pdsv;
+        cb.kv = __longlong_as_double(((long long)kv_hi << 32) | (unsigned int)kv_lo);
+      }
+      cb.n += cnt;
+      if (cb.n == 64) return true;
+    }
+  }
+};
+
+template <int C, bool MASK>
 __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ volT,
                                                      const BpView* __restrict__ views, int view0, int n_strips,
-                                                     BpGrid g, float* __restrict__ filt) {
-  extern __shared__ float strip[];  // [rows][kStripP]; column q holds padded detector column pu0 + q - 1
+                                                     BpGrid g, float* __restrict__ filt, int lds_r0, int lds_rows) {
+  // [lds_rows][kStripP] (padded rows lds_r0.., column q = padded column pu0 + q - 1), then kTrash floats
+  extern __shared__ float strip[];
   const int k = view0 + blockIdx.x / n_strips;
   const int st = blockIdx.x - (k - view0) * n_strips;
   const int pu0 = st * kStripW;      // first padded column of the strip
   const int u_lo = pu0 - kPadLo;     // first detector column index owned
   const int u_hi = u_lo + kStripW - 1;
   const int lane = threadIdx.x;
-  for (int i = lane; i < g.rows * kStripP; i += 64) strip[i] = 0.f;
+  const int trash = lds_rows * kStripP;
+  for (int i = lane; i < trash + kTrash; i += 64) strip[i] = 0.f;
   __syncthreads();
 
-  const BpView c = views[k];  // held in registers: no constant reloads inside the loops
+  const BpView& c = views[k];
   const bool x_slabs = fabs(c.cx) >= fabs(c.cy);
   // columns whose left tap iu satisfies u_lo - 1 <= iu <= u_hi, i.e. su in [u_lo - 1, u_hi + 1)
   const double s_lo = (double)(u_lo - 1) - 0.5, s_hi = (double)(u_hi + 1) + 0.5;
   const int n_slabs = x_slabs ? (g.x1 - g.x0) : g.ny;
-  for (int sl = 0; sl < n_slabs; ++sl) {
-    int a, b;
-    int xs = 0, ys = 0;
-    if (x_slabs) {
-      xs = g.x0 + sl;
-      slab_range(s_lo, s_hi, c, voxel_x(g, xs), true, g.gy0, g.gsy, g.ny, a, b);
-    } else {
-      ys = sl;
-      slab_range(s_lo, s_hi, c, voxel_y(g, ys), false, g.gx0, g.gsx, g.nx, a, b);
-      a = max(a, g.x0);
-      b = min(b, g.x1 - 1);
+  const int nch = (g.nz + C - 1) / C;
+  const int npass = (nch + 63) / 64;
+  constexpr int NP = ColumnVals<C>::NP;
+
+  Enumerator en;
+  ColumnBatch cb;
+  while (en.fill(cb, c, g, x_slabs, n_slabs, s_lo, s_hi, u_lo, u_hi, lane)) {
+    ColumnVals<C> pre;
+    {
+      const int x = bcast_i(cb.x, 0), y = bcast_i(cb.y, 0);
+      load_column<C, MASK>(pre, volT + ((size_t)y * g.nx + x) * g.nz, x, y, g, lane);
     }
-    for (int o0 = a; o0 <= b; o0 += 64) {
-      const int o = min(o0 + lane, b);
-      const int x = x_slabs ? xs : o, y = x_slabs ? o : ys;
-      const double X = voxel_x(g, x), Y = voxel_y(g, y);
-      const ColumnCore mine = column_core(c, X, Y, (float)X, (float)Y, g);
-      const bool keep = o0 + lane <= b && mine.ok && mine.iu >= u_lo - 1 && mine.iu <= u_hi;
-      uint64_t m = __ballot(keep);
-      while (m) {
-        const int src = __builtin_ctzll(m);
-        m &= m - 1;
-        ColumnCore cc;
-        cc.iu = bcast_i(mine.iu, src);
-        cc.fu = bcast_f(mine.fu, src);
-        cc.w = bcast_f(mine.w, src);
-        cc.ok = true;
-        cc.kv = bcast_d(mine.kv, src);
-        cc.dsv = bcast_f(mine.dsv, src);
-        const int oc = bcast_i(o, src);
-        const int cx = x_slabs ? xs : oc, cy = x_slabs ? oc : ys;
-        const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: -1 .. kStripW - 1
-        const float* col = volT + ((size_t)cy * g.nx + cx) * g.nz;
-        project_column<C>(strip, col, c, cc, t0, cx, cy, g, lane);
+    for (int i = 0; i < cb.n; ++i) {
+      ColumnCore cc;
+      cc.iu = bcast_i(cb.iu, i);
+      cc.fu = bcast_f(cb.fu, i);
+      cc.w = bcast_f(cb.w, i);
+      cc.ok = true;
+      cc.kv = bcast_d(cb.kv, i);
+      cc.dsv = bcast_f(cb.dsv, i);
+      const int x = bcast_i(cb.x, i), y = bcast_i(cb.y, i);
+      const ColumnVals<C> vals = pre;
+      if (i + 1 < cb.n) {  // fetch the next column's values while this one is accumulated
+        const int xn = bcast_i(cb.x, i + 1), yn = bcast_i(cb.y, i + 1);
+        load_column<C, MASK>(pre, volT + ((size_t)yn * g.nx + xn) * g.nz, xn, yn, g, lane);
+      }
+      const float step = (float)C * fabsf(cc.dsv);
+      const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
+      const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: -1 .. kStripW - 1
+      const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + 1;
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        if (p < npass) project_pass<C>(strip, vals.v[p], c, cc, cell0, trash + 2 * lane, p * 64, phases, g, lane);
+      const float* col = volT + ((size_t)y * g.nx + x) * g.nz;
+      for (int p = NP; p < npass; ++p) {  // columns longer than the prefetch window load in place
+        float v[C];
+        load_chunk<C, MASK>(v, col, p, x, y, g, lane);
+        project_pass<C>(strip, v, c, cc, cell0, trash + 2 * lane, p * 64, phases, g, lane);
       }
     }
   }
@@ -663,7 +780,9 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
   float* out = filt + (size_t)k * g.rows * g.pitch;
   for (int i = lane; i < g.rows * kStripW; i += 64) {
     const int r = i / kStripW, t = i - r * kStripW;
-    if (pu0 + t < g.pitch) out[(size_t)r * g.pitch + pu0 + t] = strip[r * kStripP + t + 1];
+    const int lr = r - lds_r0;
+    if (pu0 + t < g.pitch)
+      out[(size_t)r * g.pitch + pu0 + t] = (lr >= 0 && lr < lds_rows) ? strip[lr * kStripP + t + 1] : 0.f;
   }
 }
 
@@ -691,12 +810,13 @@ hipError_t launch_transpose_zx(const float* vol, float* volT, int nx, int ny, in
 }
 
 hipError_t launch_project(const float* volT, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
-                          float* filt_pad, const monte_tuning& t, hipStream_t s) {
+                          float* filt_pad, int lds_r0, int lds_rows, const monte_tuning& t, hipStream_t s) {
   const int n_strips = (grid.pitch + kStripW - 1) / kStripW;
   const long nblocks = (long)n_strips * nviews_chunk;
   if (nblocks <= 0) return hipSuccess;
   if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
-  const size_t lds = sizeof(float) * (size_t)grid.rows * kStripP;
+  if (lds_r0 < 0 || lds_rows < 1 || lds_r0 + lds_rows > grid.rows) return hipErrorInvalidValue;
+  const size_t lds = sizeof(float) * ((size_t)lds_rows * kStripP + kTrash);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   // voxels per lane: fill the wave's 64 lanes best over nz, favouring short chunks
   int best = t.pj_zchunk;
@@ -708,12 +828,17 @@ hipError_t launch_project(const float* volT, const BpView* views, int view0, int
     if (best_cost < 0 || cost < best_cost) { best = cz; best_cost = cost; }
   }
   const dim3 gd((unsigned)nblocks), bd(64);
+#define MONTE_PJ(CZ, MK)                                                                                      \
+  hipLaunchKernelGGL((project_kernel<CZ, MK>), gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad, \
+                     lds_r0, lds_rows)
+  const bool mk = grid.mask != 0;
   switch (best) {
-    case 2: hipLaunchKernelGGL(project_kernel<2>, gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad); break;
-    case 4: hipLaunchKernelGGL(project_kernel<4>, gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad); break;
-    case 8: hipLaunchKernelGGL(project_kernel<8>, gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad); break;
+    case 2: if (mk) MONTE_PJ(2, true); else MONTE_PJ(2, false); break;
+    case 4: if (mk) MONTE_PJ(4, true); else MONTE_PJ(4, false); break;
+    case 8: if (mk) MONTE_PJ(8, true); else MONTE_PJ(8, false); break;
     default: return hipErrorInvalidValue;
   }
+#undef MONTE_PJ
   return hipGetLastError();
 }
 

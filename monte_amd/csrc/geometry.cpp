@@ -4,6 +4,7 @@
 // coordinates from it in FP64); the distance-weight terms and the filter
 // constants are handed over in FP32.  One set of kernels serves both presets;
 // the presets differ only in these numbers (see BpView in internal.h).
+#include <algorithm>
 #include <cmath>
 
 #include "internal.h"
@@ -143,6 +144,38 @@ void compute_filter_taps(const monte_geom& g, std::vector<float>& h) {
     }
     h[i + n - 1] = (float)v;
   }
+}
+
+// The v sample coordinate of a voxel is sv = sv0 + svk Z / U with U affine in
+// (X, Y) and positive over the volume, so over the voxel-centre box its
+// extremes lie at the box's 8 corners.  Rows written are floor(sv) and the one
+// above, clamped to the v window; one row of margin each side covers rounding.
+void projector_rows(const std::vector<BpView>& views, int view0, int n, const BpGrid& grid, int& lds_r0,
+                    int& lds_rows) {
+  double lo = 1e300, hi = -1e300;
+  const double xs[2] = {grid.gx0 + grid.x0 * grid.gsx, grid.gx0 + (grid.x1 - 1) * grid.gsx};
+  const double ys[2] = {grid.gy0, grid.gy0 + (grid.ny - 1) * grid.gsy};
+  const double zs[2] = {grid.gz0, grid.gz0 + (grid.nz - 1) * grid.gsz};
+  for (int k = view0; k < view0 + n; ++k) {
+    const BpView& c = views[k];
+    for (double X : xs)
+      for (double Y : ys) {
+        const double U = c.cx * X + c.cy * Y + c.c0;
+        for (double Z : zs) {
+          const double sv = c.sv0 + c.svk * Z / U;
+          lo = std::min(lo, sv);
+          hi = std::max(hi, sv);
+        }
+      }
+  }
+  const double wlo = std::floor(grid.lo_v), whi = std::floor(grid.hi_v);
+  const double rlo = std::max(std::floor(lo) - 2.0, wlo), rhi = std::min(std::floor(hi) + 2.0, whi) + 1.0;
+  int r0 = (int)rlo + kPadLo, r1 = (int)rhi + kPadLo;  // padded rows, inclusive
+  r0 = std::max(r0, 0);
+  r1 = std::min(r1, grid.rows - 1);
+  if (r1 < r0) r1 = r0;
+  lds_r0 = r0;
+  lds_rows = r1 - r0 + 1;
 }
 
 }  // namespace monte

@@ -187,9 +187,14 @@ hipError_t launch_fold_pads(float* buf, int n_views, int nu, int nv, int pitch, 
 hipError_t launch_backproject(const float* filt_pad, const BpView* views, int view0, int nviews_chunk,
                               const BpGrid& grid, float* vol, int accumulate, const monte_tuning& t, hipStream_t s);
 // volT: the volume with z fastest ([y][x][z], see launch_transpose_zx); every
-// cell of the padded views [view0, view0 + nviews_chunk) is written.
+// cell of the padded views [view0, view0 + nviews_chunk) is written.  Padded
+// rows [lds_r0, lds_r0 + lds_rows) must hold every row the views' voxels can
+// reach (projector_rows); the others are written as zeros.
 hipError_t launch_project(const float* volT, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
-                          float* filt_pad, const monte_tuning& t, hipStream_t s);
+                          float* filt_pad, int lds_r0, int lds_rows, const monte_tuning& t, hipStream_t s);
+// Padded detector rows reachable by the voxels of views [view0, view0 + n).
+void projector_rows(const std::vector<BpView>& views, int view0, int n, const BpGrid& grid, int& lds_r0,
+                    int& lds_rows);
 hipError_t launch_transpose_zx(const float* vol, float* volT, int nx, int ny, int nz, hipStream_t s);
 hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* taps, const FilterParams& fp,
                                hipStream_t s);
