@@ -519,34 +519,49 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 // wave's LDS operations execute in program order, so updates of a cell by
 // neighbouring lanes, later voxels and later columns all see each other.
 // Strip columns 0 and kStripW + 1 are guards that absorb taps landing just
-// outside the strip; the row pitch is odd so strided rows spread over banks.
-constexpr int kStripW = 7;
-constexpr int kStripP = kStripW + 2;  // odd
+// outside the strip.  The row pitch sets how the lanes' strided rows fall on
+// LDS banks: at 325^3 / 512^2 a pitch of 7 ran 20% faster than 9 and 30%
+// faster than the even pitches 6 and 10, and a 5-column strip balances the
+// per-strip enumeration against the waves a CU can hold.
+constexpr int kStripW = 5;
+constexpr int kStripP = kStripW + 2;
 
-__device__ inline void slab_range(double s_lo, double s_hi, const BpView& c, double coord, bool x_slabs,
-                                  double o0, double os, int n, int& a, int& b) {
-  // invert su = su0 + suk (ax X + ay Y) / (cx X + cy Y + c0) for the other coordinate
-  double lo = 1e300, hi = -1e300;
-  bool whole = false;
-  for (int e = 0; e < 2; ++e) {
-    const double q = ((e ? s_hi : s_lo) - c.su0) / c.suk;
-    double num, den;
-    if (x_slabs) {  // X fixed, solve Y
-      num = q * (c.cx * coord + c.c0) - c.ax * coord;
-      den = c.ay - q * c.cy;
-    } else {        // Y fixed, solve X
-      num = q * (c.cy * coord + c.c0) - c.ay * coord;
-      den = c.ax - q * c.cx;
-    }
-    if (fabs(den) < 1e-12) { whole = true; break; }
-    const double idx = (num / den - o0) / os;  // fractional voxel index
-    lo = fmin(lo, idx);
-    hi = fmax(hi, idx);
+// Columns of one strip on each slab.  The strip's two edges su = s_lo and
+// su = s_hi are lines through the source; on a slab (x fixed for x-slabs, y
+// fixed for y-slabs) each crosses the other coordinate at
+//   t = (q (c1 coord + c0) - a1 coord) / (a2 - q c2),  q = (s - su0) / suk,
+// with (c1, c2, a1, a2) = (cx, cy, ax, ay) for x-slabs and (cy, cx, ay, ax)
+// for y-slabs.  The denominators depend only on the view and the edge, so the
+// per-slab work is two multiply-adds per edge.  The range is widened by one
+// voxel each side; every column in it is then tested exactly.
+struct SlabRanges {
+  double q0, q1, id0, id1;  // edge parameters and reciprocal denominators
+  double o0, ios;           // voxel index = (t - o0) * ios along the slab
+  int n;                    // voxels along the slab
+  bool whole;               // an edge runs parallel to the slabs: take every column
+
+  __device__ void init(double s_lo, double s_hi, const BpView& c, bool x_slabs, const BpGrid& g) {
+    q0 = (s_lo - c.su0) / c.suk;
+    q1 = (s_hi - c.su0) / c.suk;
+    const double a2 = x_slabs ? c.ay : c.ax, c2 = x_slabs ? c.cy : c.cx;
+    const double d0 = a2 - q0 * c2, d1 = a2 - q1 * c2;
+    whole = fabs(d0) < 1e-12 || fabs(d1) < 1e-12;
+    id0 = 1.0 / d0;
+    id1 = 1.0 / d1;
+    o0 = x_slabs ? g.gy0 : g.gx0;
+    ios = 1.0 / (x_slabs ? g.gsy : g.gsx);
+    n = x_slabs ? g.ny : g.nx;
   }
-  if (whole) { a = 0; b = n - 1; return; }
-  a = (int)fmax(floor(lo) - 1.0, 0.0);
-  b = (int)fmin(ceil(hi) + 1.0, (double)(n - 1));
-}
+  __device__ void range(const BpView& c, bool x_slabs, double coord, int& a, int& b) const {
+    if (whole) { a = 0; b = n - 1; return; }
+    const double c1 = x_slabs ? c.cx : c.cy, a1 = x_slabs ? c.ax : c.ay;
+    const double base = c1 * coord + c.c0;
+    const double t0 = (q0 * base - a1 * coord) * id0, t1 = (q1 * base - a1 * coord) * id1;
+    const double i0 = (t0 - o0) * ios, i1 = (t1 - o0) * ios;
+    a = (int)fmax(floor(fmin(i0, i1)) - 1.0, 0.0);
+    b = (int)fmin(ceil(fmax(i0, i1)) + 1.0, (double)(n - 1));
+  }
+};
 
 __device__ inline int bcast_i(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
 __device__ inline float bcast_f(float v, int src) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src)); }
@@ -563,10 +578,13 @@ __device__ inline void lds_order() { asm volatile("" ::: "memory"); }
 
 // Passes (64 chunks of C voxels) whose volume values are fetched one column
 // ahead; columns longer than kPrefetchZ / C / 64 passes load the rest in place.
-constexpr int kPrefetchZ = 512;
+constexpr int kPrefetchZ = 384;
+constexpr int kAhead = 4;
 // LDS floats after the strip that absorb the updates of lanes with nothing to
 // add (outside the v window or past the column end), so the updates need no
-// branches: two per lane, one row pitch and one cell of overlap.
+// branches: two per lane, one row pitch and one cell of overlap.  (Branches
+// around the updates, or handing the upper-row update to the next lane by a
+// DPP shift, both measured slower.)
 constexpr int kTrash = 2 * 64 + kStripP + 2;
 
 template <int C>
@@ -598,13 +616,14 @@ __device__ inline void load_chunk(float (&v)[C], const float* __restrict__ col, 
 
 template <int C, bool MASK>
 __device__ inline void load_column(ColumnVals<C>& cv, const float* __restrict__ col, int x, int y, const BpGrid& g,
-                                   int lane) {
+                                   int lane, int npass) {
 #pragma unroll
-  for (int p = 0; p < ColumnVals<C>::NP; ++p) load_chunk<C, MASK>(cv.v[p], col, p, x, y, g, lane);
+  for (int p = 0; p < ColumnVals<C>::NP; ++p)
+    if (p < npass) load_chunk<C, MASK>(cv.v[p], col, p, x, y, g, lane);
 }
 
 // The C voxel updates of one lane; lanes with `act` false (or nothing to add)
-// update the trash floats instead.
+// update their trash floats instead.
 template <int C>
 __device__ inline void pass_updates(float* __restrict__ strip, const float (&v)[C], const ColumnCore& cc,
                                     const ChunkBase& cb, int joff, int z0, bool act, int cell0, int trash,
@@ -660,41 +679,46 @@ __device__ inline float permute_f(int dst, float v) {
   return __int_as_float(__builtin_amdgcn_ds_permute(dst << 2, __float_as_int(v)));
 }
 
-// Walks the strip's slab ranges and appends the kept columns of each range to
-// the batch (each kept lane pushes its column to lane n + its rank; the other
-// lanes push to the lanes past the new end), until the batch is full or the
-// slabs are exhausted.  Returns false when nothing is left.
+// Walks the strip's slabs four at a time, 16 lanes per slab, and appends the
+// kept columns to the batch (each kept lane pushes its column to lane n + its
+// rank; the other lanes push to the lanes past the new end), until the batch
+// is full or the slabs are exhausted.  Returns false when nothing is left.
 struct Enumerator {
-  int sl = 0, o0 = 0, a = 0, b = -1, xs = 0, ys = 0;
+  SlabRanges sr;
+  int s0 = 0;   // first slab of the current group of four
+  int rnd = 0;  // 16-column round within the group
 
-  __device__ bool fill(ColumnBatch& cb, const BpView& c, const BpGrid& g, bool x_slabs, int n_slabs, double s_lo,
-                       double s_hi, int u_lo, int u_hi, int lane) {
+  __device__ bool fill(ColumnBatch& cb, const BpView& c, const BpGrid& g, bool x_slabs, int n_slabs, int u_lo,
+                       int u_hi, int lane) {
     cb.n = 0;
     for (;;) {
-      if (o0 > b) {
-        if (sl >= n_slabs) return cb.n > 0;
-        if (x_slabs) {
-          xs = g.x0 + sl;
-          slab_range(s_lo, s_hi, c, voxel_x(g, xs), true, g.gy0, g.gsy, g.ny, a, b);
-        } else {
-          ys = sl;
-          slab_range(s_lo, s_hi, c, voxel_y(g, ys), false, g.gx0, g.gsx, g.nx, a, b);
+      if (s0 >= n_slabs) return cb.n > 0;
+      const int sl = s0 + (lane >> 4);
+      int a = 0, b = -1;
+      if (sl < n_slabs) {
+        const int sc = x_slabs ? g.x0 + sl : sl;
+        sr.range(c, x_slabs, x_slabs ? voxel_x(g, sc) : voxel_y(g, sc), a, b);
+        if (!x_slabs) {
           a = max(a, g.x0);
           b = min(b, g.x1 - 1);
         }
-        ++sl;
-        o0 = a;
-        continue;
       }
-      const int o = min(o0 + lane, b);
-      const int x = x_slabs ? xs : o, y = x_slabs ? o : ys;
+      const int w = max(b - a + 1, 0);
+      const int need = max(max(bcast_i(w, 0), bcast_i(w, 16)), max(bcast_i(w, 32), bcast_i(w, 48)));
+      const int off = a + rnd * 16 + (lane & 15);
+      const int o = min(off, max(b, a));
+      const int xo = x_slabs ? g.x0 + sl : o, yo = x_slabs ? o : sl;
+      const int x = min(xo, g.x1 - 1), y = min(yo, g.ny - 1);
       const double X = voxel_x(g, x), Y = voxel_y(g, y);
       const ColumnCore mine = column_core(c, X, Y, (float)X, (float)Y, g);
-      const bool keep = o0 + lane <= b && mine.ok && mine.iu >= u_lo - 1 && mine.iu <= u_hi;
+      const bool keep = sl < n_slabs && off <= b && mine.ok && mine.iu >= u_lo - 1 && mine.iu <= u_hi;
       const uint64_t m = __ballot(keep);
       const int cnt = (int)__popcll(m);
-      if (cb.n + cnt > 64) return true;  // full: this range is taken again by the next fill
-      o0 += 64;
+      if (cb.n + cnt > 64) return true;  // full: this round is taken again by the next fill
+      if (++rnd * 16 >= need) {
+        rnd = 0;
+        s0 += 4;
+      }
       if (!cnt) continue;
       const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
       const int dst = keep ? cb.n + (int)rk : (cb.n + cnt + (lane - (int)rk)) & 63;
@@ -740,39 +764,53 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
   constexpr int NP = ColumnVals<C>::NP;
 
   Enumerator en;
+  en.sr.init(s_lo, s_hi, c, x_slabs, g);
   ColumnBatch cb;
-  while (en.fill(cb, c, g, x_slabs, n_slabs, s_lo, s_hi, u_lo, u_hi, lane)) {
-    ColumnVals<C> pre;
-    {
-      const int x = bcast_i(cb.x, 0), y = bcast_i(cb.y, 0);
-      load_column<C, MASK>(pre, volT + ((size_t)y * g.nx + x) * g.nz, x, y, g, lane);
-    }
-    for (int i = 0; i < cb.n; ++i) {
-      ColumnCore cc;
-      cc.iu = bcast_i(cb.iu, i);
-      cc.fu = bcast_f(cb.fu, i);
-      cc.w = bcast_f(cb.w, i);
-      cc.ok = true;
-      cc.kv = bcast_d(cb.kv, i);
-      cc.dsv = bcast_f(cb.dsv, i);
-      const int x = bcast_i(cb.x, i), y = bcast_i(cb.y, i);
-      const ColumnVals<C> vals = pre;
-      if (i + 1 < cb.n) {  // fetch the next column's values while this one is accumulated
-        const int xn = bcast_i(cb.x, i + 1), yn = bcast_i(cb.y, i + 1);
-        load_column<C, MASK>(pre, volT + ((size_t)yn * g.nx + xn) * g.nz, xn, yn, g, lane);
-      }
-      const float step = (float)C * fabsf(cc.dsv);
-      const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
-      const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: -1 .. kStripW - 1
-      const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + 1;
+  // The volume values of the next kAhead columns are in flight while one is
+  // accumulated: a column's accumulation takes less time than a load from HBM.
+  auto fetch = [&](ColumnVals<C>& dst, int i) {
+    const int x = bcast_i(cb.x, i), y = bcast_i(cb.y, i);
+    load_column<C, MASK>(dst, volT + ((size_t)y * g.nx + x) * g.nz, x, y, g, lane, npass);
+  };
+  auto accumulate = [&](const ColumnVals<C>& vals, int i) {
+    ColumnCore cc;
+    cc.iu = bcast_i(cb.iu, i);
+    cc.fu = bcast_f(cb.fu, i);
+    cc.w = bcast_f(cb.w, i);
+    cc.ok = true;
+    cc.kv = bcast_d(cb.kv, i);
+    cc.dsv = bcast_f(cb.dsv, i);
+    const float step = (float)C * fabsf(cc.dsv);
+    const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
+    const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: -1 .. kStripW - 1
+    const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + 1;
 #pragma unroll
-      for (int p = 0; p < NP; ++p)
-        if (p < npass) project_pass<C>(strip, vals.v[p], c, cc, cell0, trash + 2 * lane, p * 64, phases, g, lane);
+    for (int p = 0; p < NP; ++p)
+      if (p < npass) project_pass<C>(strip, vals.v[p], c, cc, cell0, trash + 2 * lane, p * 64, phases, g, lane);
+    if (npass > NP) {  // columns longer than the prefetch window load the rest in place
+      const int x = bcast_i(cb.x, i), y = bcast_i(cb.y, i);
       const float* col = volT + ((size_t)y * g.nx + x) * g.nz;
-      for (int p = NP; p < npass; ++p) {  // columns longer than the prefetch window load in place
+      for (int p = NP; p < npass; ++p) {
         float v[C];
         load_chunk<C, MASK>(v, col, p, x, y, g, lane);
         project_pass<C>(strip, v, c, cc, cell0, trash + 2 * lane, p * 64, phases, g, lane);
+      }
+    }
+  };
+  while (en.fill(cb, c, g, x_slabs, n_slabs, u_lo, u_hi, lane)) {
+    ColumnVals<C> ring[kAhead];
+#pragma unroll
+    for (int q = 0; q < kAhead; ++q)
+      if (q < cb.n) fetch(ring[q], q);
+    for (int i0 = 0; i0 < cb.n; i0 += kAhead) {
+#pragma unroll
+      for (int q = 0; q < kAhead; ++q) {
+        const int i = i0 + q;
+        if (i < cb.n) {
+          const ColumnVals<C> vals = ring[q];
+          if (i + kAhead < cb.n) fetch(ring[q], i + kAhead);
+          accumulate(vals, i);
+        }
       }
     }
   }
