@@ -145,7 +145,21 @@ __device__ inline bool ray_box(float px, float py, float pz, float dx, float dy,
   return true;
 }
 
-__device__ inline int job_of_unit(const McJob* __restrict__ jobs, int n_jobs, uint64_t u) {
+using CJob = const __attribute__((address_space(4))) McJob;
+using CU32 = const __attribute__((address_space(4))) uint32_t;
+
+// Job i read through the constant address space (scalar loads).
+__device__ inline McJob load_job(const CJob* jobs, int i) {
+  static_assert(sizeof(McJob) % 4 == 0, "McJob is copied as dwords");
+  McJob j;
+  CU32* src = (CU32*)(jobs + i);
+  uint32_t* dst = (uint32_t*)&j;
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(McJob) / 4); ++k) dst[k] = src[k];
+  return j;
+}
+
+__device__ inline int job_of_unit(const CJob* jobs, int n_jobs, uint64_t u) {
   int lo = 0, hi = n_jobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -167,6 +181,15 @@ struct Photon {
   int lab;             // label at the pending collision
   int nint;            // real interactions so far
   int view;            // tally view index (absolute)
+  float cv, sv;        // cos, sin of the view angle
+};
+
+// Per-wave LDS copy of the current job's per-energy tables, read at every
+// birth and flight start (the spectrum search alone is 8 dependent reads).
+struct WaveTables {
+  float cdf[kRows];      // spectrum CDF (unused for mono jobs)
+  float mumax[kRows];    // Woodcock majorant
+  float mu[4][kRows];    // mu of labels 0..3
 };
 
 enum { kDead = 0, kFly = 1, kCollide = 2 };
@@ -174,9 +197,9 @@ enum { kTallyMiss = 0, kTallyPrimary = 1, kTallyScatter = 2 };
 
 // Straight line from the current position to the detector plane x' = sdd - sod.
 __device__ inline int tally(const Photon& p, const McJob& J) {
-  const float2 cs = J.view_cs[p.view];
-  const float xq = p.px * cs.x + p.py * cs.y, yq = -p.px * cs.y + p.py * cs.x;
-  const float ddx = p.dx * cs.x + p.dy * cs.y, ddy = -p.dx * cs.y + p.dy * cs.x;
+  const float cx = p.cv, cy = p.sv;
+  const float xq = p.px * cx + p.py * cy, yq = -p.px * cy + p.py * cx;
+  const float ddx = p.dx * cx + p.dy * cy, ddy = -p.dx * cy + p.dy * cx;
   if (!(ddx > 0.f)) return kTallyMiss;
   const float tt = ((J.sdd - J.sod) - xq) / ddx;
   const float yh = fmaf(tt, ddy, yq), zh = fmaf(tt, p.dz, p.pz);
@@ -196,15 +219,15 @@ __device__ inline int tally(const Photon& p, const McJob& J) {
 
 // Starts a flight from the current position; returns false if the photon
 // leaves without entering the phantom box (caller tallies it).
-__device__ inline bool start_flight(Photon& p, const McJob& J) {
+__device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables& wt) {
   float t0, t1;
   if (!ray_box(p.px, p.py, p.pz, p.dx, p.dy, p.dz, J, t0, t1) || !(t1 > 0.f)) return false;
-  const float mm = J.tables->mumax[p.eb];
+  const float mm = wt.mumax[p.eb];
   if (!(mm > 0.f)) return false;
-  p.mu0 = J.tables->m[0][p.eb].x;
-  p.mu1 = J.tables->m[1][p.eb].x;
-  p.mu2 = J.tables->m[2][p.eb].x;
-  p.mu3 = J.tables->m[3][p.eb].x;
+  p.mu0 = wt.mu[0][p.eb];
+  p.mu1 = wt.mu[1][p.eb];
+  p.mu2 = wt.mu[2][p.eb];
+  p.mu3 = wt.mu[3][p.eb];
   if (t0 > 0.f) {
     p.px = fmaf(t0, p.dx, p.px); p.py = fmaf(t0, p.dy, p.py); p.pz = fmaf(t0, p.dz, p.pz);
     t1 -= t0;
@@ -214,6 +237,16 @@ __device__ inline bool start_flight(Photon& p, const McJob& J) {
   p.mumax = mm;
   p.inv_mumax = 1.0f / mm;
   return true;
+}
+
+// Copies a job's tables into the wave's LDS slot.
+__device__ inline void load_wave_tables(WaveTables& wt, const McJob& J, int lane) {
+  for (int e = lane; e < kRows; e += 64) {
+    wt.cdf[e] = J.spectrum_cdf ? J.spectrum_cdf[e] : 0.f;
+    wt.mumax[e] = J.tables->mumax[e];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) wt.mu[l][e] = J.tables->m[l][e].x;
+  }
 }
 
 // Attenuation at a label: labels 0..3 from registers, others from the table.
@@ -276,10 +309,16 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   Photon p{};
   Stream rs{};
   WaveCounters cnt{};
-  const McJob* Jp = jobs;
   const int lane = __lane_id();
+  // The job descriptors do not change during the launch: read them through the
+  // constant address space, so wave-uniform fields come in by scalar loads.
+  const CJob* cjobs = (const CJob*)jobs;
+  extern __shared__ WaveTables wave_tables[];  // one slot per wave of the block
+  WaveTables& wt = wave_tables[threadIdx.x >> 6];
 
   for (;;) {
+    const McJob J = load_job(cjobs, __builtin_amdgcn_readfirstlane(job < 0 ? 0 : job));
+    const McJob* Jp = &J;
     uint64_t dead = __ballot(state == kDead);
     const uint64_t fly = __ballot(state == kFly);
     const uint64_t coll = __ballot(state == kCollide);
@@ -290,24 +329,30 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       if (pending < 0 && !exhausted) {
         unsigned int u = 0;
         if (lane == 0) u = atomicAdd(queue, 1u);
-        u = __shfl(u, 0, 64);
+        u = (unsigned int)__builtin_amdgcn_readfirstlane((int)u);
         if ((uint64_t)u >= total_units) exhausted = true; else pending = (long long)u;
       }
       if (pending >= 0) {
-        const int pj = __builtin_amdgcn_readfirstlane(job_of_unit(jobs, n_jobs, (uint64_t)pending));
+        const int pj = __builtin_amdgcn_readfirstlane(job_of_unit(cjobs, n_jobs, (uint64_t)pending));
         if (pj == job || all_dead) {
-          if (pj != job) {
+          const bool switched = pj != job;
+          if (switched) {
             if (job >= 0) flush_counters(cnt, counters + job);
             job = pj;
-            Jp = jobs + job;
           }
-          const uint64_t local = (uint64_t)pending - Jp->unit_begin;
-          const uint32_t vl = (uint32_t)(local / Jp->batches_per_view);
-          const uint32_t bi = (uint32_t)(local - (uint64_t)vl * Jp->batches_per_view);
-          unit_view = Jp->view_begin + (int)vl;
-          next = bi * (uint32_t)batch;
-          end = min(next + (uint32_t)batch, Jp->photons_per_view);
+          const uint64_t local = (uint64_t)pending - cjobs[pj].unit_begin;
+          const uint32_t bpv = cjobs[pj].batches_per_view;
+          const uint32_t vl = (uint32_t)(local / bpv);
+          const uint32_t bi = (uint32_t)(local - (uint64_t)vl * bpv);
+          unit_view = __builtin_amdgcn_readfirstlane(cjobs[pj].view_begin + (int)vl);
+          next = (uint32_t)__builtin_amdgcn_readfirstlane((int)(bi * (uint32_t)batch));
+          end = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(next + (uint32_t)batch, cjobs[pj].photons_per_view));
           pending = -1;
+          if (switched) {  // J is re-read for the new job at the top of the loop
+            load_wave_tables(wt, load_job(cjobs, pj), lane);
+            __builtin_amdgcn_wave_barrier();
+            continue;
+          }
         }
       }
     }
@@ -331,8 +376,11 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           const float yp = ((float)iu + 0.5f - Jp->un0) * Jp->u_step;
           const float zp = ((float)iv + 0.5f - Jp->vn0) * Jp->v_step;
           const float in = rsqrtf(Jp->sdd * Jp->sdd + yp * yp + zp * zp);
-          const float2 cs = Jp->view_cs[unit_view];
+          CU32* vcs = (CU32*)Jp->view_cs + 2 * unit_view;  // read-only during the launch: scalar loads
+          const float2 cs = make_float2(__uint_as_float(vcs[0]), __uint_as_float(vcs[1]));
           p.view = unit_view;
+          p.cv = cs.x;
+          p.sv = cs.y;
           p.dx = (Jp->sdd * cs.x - yp * cs.y) * in;
           p.dy = (Jp->sdd * cs.y + yp * cs.x) * in;
           p.dz = zp * in;
@@ -343,14 +391,14 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             int lo = 1, hi = kRows - 1;
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
-              if (Jp->spectrum_cdf[mid] < u) lo = mid + 1; else hi = mid;
+              if (wt.cdf[mid] < u) lo = mid + 1; else hi = mid;
             }
             e = (float)lo;
           }
           p.e = e;
           p.eb = ebin_of(e);
           p.nint = 0;
-          if (start_flight(p, *Jp)) {
+          if (start_flight(p, *Jp, wt)) {
             state = kFly;
           } else {
             code = tally(p, *Jp);
@@ -360,7 +408,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       }
       cnt.histories += popc(__ballot(born));
       count_tally(cnt, tallied, code);
-      next = min(end, next + popc(dead));
+      next = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(end, next + popc(dead)));
     }
 
     // 3) interactions of the parked colliding lanes
@@ -398,7 +446,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             rotate_dir(p.dx, p.dy, p.dz, cth, phi);
           }
           p.nint += 1;
-          if (start_flight(p, *Jp)) {
+          if (start_flight(p, *Jp, wt)) {
             state = kFly;
           } else {
             code = tally(p, *Jp);
@@ -477,8 +525,8 @@ hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs
 hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, unsigned int* queue,
                                McCounters* counters, int batch, int block, int grid, hipStream_t s) {
   if (block % 64 || block > 256 || block <= 0 || grid <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(mc_transport_kernel, dim3(grid), dim3(block), 0, s, jobs_dev, n_jobs, total_units, queue,
-                     counters, batch);
+  hipLaunchKernelGGL(mc_transport_kernel, dim3(grid), dim3(block), sizeof(WaveTables) * (block / 64), s, jobs_dev,
+                     n_jobs, total_units, queue, counters, batch);
   return hipGetLastError();
 }
 
