@@ -31,8 +31,10 @@ __device__ inline void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32
                                 uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    // one 32 x 32 -> 64-bit multiply gives both halves (v_mad_u64_u32)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     const uint32_t n0 = hi1 ^ c1 ^ k0;
     const uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
@@ -175,7 +177,7 @@ struct Photon {
   float e;             // energy (keV)
   float t, t1;         // flight parameter and box-exit parameter
   float mumax;         // Woodcock majorant of the current table row
-  float inv_mumax;
+  float lscale;        // ln 2 / mumax: a free path is -log2(u) * lscale
   float mu0, mu1, mu2, mu3;  // mu of labels 0..3 at the current table row
   int eb;              // table row
   int lab;             // label at the pending collision
@@ -235,7 +237,7 @@ __device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables&
   p.t = 0.f;
   p.t1 = t1;
   p.mumax = mm;
-  p.inv_mumax = 1.0f / mm;
+  p.lscale = 0.69314718055994531f / mm;
   return true;
 }
 
@@ -263,7 +265,9 @@ __device__ inline int label_at(const Photon& p, float t, const McJob& J, float& 
   const int ix = min(max((int)floorf((qx - J.bx0) * J.inv_vx), 0), J.nx - 1);
   const int iy = min(max((int)floorf((qy - J.by0) * J.inv_vy), 0), J.ny - 1);
   const int iz = min(max((int)floorf((qz - J.bz0) * J.inv_vz), 0), J.nz - 1);
-  return J.labels[((size_t)iz * J.ny + iy) * J.nx + ix];
+  // 32-bit voxel index (checked on the host) and a global-address-space load
+  const uint32_t i = ((uint32_t)iz * (uint32_t)J.ny + (uint32_t)iy) * (uint32_t)J.nx + (uint32_t)ix;
+  return ((const __attribute__((address_space(1))) uint8_t*)J.labels)[i];
 }
 
 // Wave-uniform statistics (scalar registers), flushed per job.
@@ -475,8 +479,9 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       philox10(c0, c1, c2, c3, Jp->key0, Jp->key1);
       rs.ctr += 1;
       rs.left = 0;
-      const float ta = fmaf(-__logf(u01(c0)), p.inv_mumax, p.t);
-      const float tb = fmaf(-__logf(u01(c2)), p.inv_mumax, ta);
+      // u >= 2^-25, so the hardware log2 needs no denormal handling
+      const float ta = fmaf(-__builtin_amdgcn_logf(u01(c0)), p.lscale, p.t);
+      const float tb = fmaf(-__builtin_amdgcn_logf(u01(c2)), p.lscale, ta);
       float ax, ay, az, bx, by, bz;
       const int la = label_at(p, ta, *Jp, ax, ay, az);
       const int lb = label_at(p, tb, *Jp, bx, by, bz);
