@@ -108,7 +108,8 @@ typedef struct monte_phantom {
 } monte_phantom;
 
 typedef struct monte_mc_cfg {
-  uint64_t photons_per_view; /* aimed round-robin at pixel centres (CBCT_real325.cu:452-466) */
+  uint64_t photons_per_view; /* photon n aims at the centre of pixel n mod (nu nv) taken in the order
+                                iv * nu + iu (CBCT_real325.cu:452-466 aims at pixel centres) */
   double e_kev;              /* mono-energetic source when spectrum == NULL */
   const double *spectrum;    /* HOST pointer: photon weight per keV bin [201], or NULL */
   int32_t max_interactions;  /* reference ScatterNUM = 5 (CBCT_real325.cu:9) */
@@ -203,7 +204,7 @@ int monte_mc_batch(monte_ctx *ctx, const monte_mc_job *jobs, int n_jobs, monte_m
 int monte_primary_exact(monte_ctx *ctx, const monte_geom *g, const monte_phantom *ph, double e_kev,
                         const double *spectrum, float *lineint);
 /* Counts -> line integrals, monte_cu/CBCT_real325.cu:264-294:
- * ln(N0) - ln(clamp(N, 1, N0)) with N0 the photons aimed at the pixel. */
+ * ln(N0) - ln(clamp(N, 1, N0)) with N0 the photons aimed at the pixel (see photons_per_view). */
 int monte_counts_to_lineint(monte_ctx *ctx, const monte_geom *g, const double *counts,
                             uint64_t photons_per_view, float *lineint);
 

@@ -344,10 +344,12 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             if (job >= 0) flush_counters(cnt, counters + job);
             job = pj;
           }
+          // units run views fastest: the units in flight share photon batches, hence
+          // detector rows, hence a thin z-slab of the phantom that stays in L2
           const uint64_t local = (uint64_t)pending - cjobs[pj].unit_begin;
-          const uint32_t bpv = cjobs[pj].batches_per_view;
-          const uint32_t vl = (uint32_t)(local / bpv);
-          const uint32_t bi = (uint32_t)(local - (uint64_t)vl * bpv);
+          const uint32_t nvs = (uint32_t)cjobs[pj].n_views_sim;
+          const uint32_t bi = (uint32_t)(local / nvs);
+          const uint32_t vl = (uint32_t)(local - (uint64_t)bi * nvs);
           unit_view = __builtin_amdgcn_readfirstlane(cjobs[pj].view_begin + (int)vl);
           next = (uint32_t)__builtin_amdgcn_readfirstlane((int)(bi * (uint32_t)batch));
           end = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(next + (uint32_t)batch, cjobs[pj].photons_per_view));
@@ -375,8 +377,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           rs.ctr = 0; rs.photon = id; rs.view = (uint32_t)unit_view; rs.left = 0;
           // aim at the centre of pixel id mod (nu*nv) (CBCT_real325.cu:464-466)
           const uint32_t npix = (uint32_t)(Jp->nu * Jp->nv);
-          const uint32_t pix = id % npix;
-          const int iu = (int)(pix / (uint32_t)Jp->nv), iv = (int)(pix - (uint32_t)iu * (uint32_t)Jp->nv);
+          const uint32_t pix = id % npix;  // aim order: u fastest, so a batch covers whole detector rows
+          const int iv = (int)(pix / (uint32_t)Jp->nu), iu = (int)(pix - (uint32_t)iv * (uint32_t)Jp->nu);
           const float yp = ((float)iu + 0.5f - Jp->un0) * Jp->u_step;
           const float zp = ((float)iv + 0.5f - Jp->vn0) * Jp->v_step;
           const float in = rsqrtf(Jp->sdd * Jp->sdd + yp * yp + zp * zp);
@@ -656,14 +658,18 @@ hipError_t launch_primary_exact(const PrimaryGeom& P, const double* cs64, const 
 
 // ------------------------------------------------- counts -> line integrals
 // CBCT_real325.cu:267-294: clamp N to [1, N0], L = ln N0 - ln N, with N0 the
-// photons aimed at the pixel: floor(N/npix) + (pixel < N mod npix).
+// photons aimed at the pixel: floor(N/npix) + (a < N mod npix), a = iv nu + iu
+// the pixel's place in the aim order.
 __global__ void counts_to_lineint_kernel(const double* __restrict__ counts, float* __restrict__ out, int n_views,
-                                         int npix, uint64_t photons_per_view) {
+                                         int nu, int nv, uint64_t photons_per_view) {
+  const int npix = nu * nv;
   const size_t n = (size_t)n_views * npix;
   const uint64_t base = photons_per_view / (uint64_t)npix, rem = photons_per_view % (uint64_t)npix;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int pix = (int)(i % (size_t)npix);
-    const double n0 = (double)(base + ((uint64_t)pix < rem ? 1 : 0));
+    const int pix = (int)(i % (size_t)npix);  // iu nv + iv
+    const int iu = pix / nv, iv = pix - iu * nv;
+    const int a = iv * nu + iu;
+    const double n0 = (double)(base + ((uint64_t)a < rem ? 1 : 0));
     double c = counts[i];
     c = c > n0 ? n0 : c;
     c = c < 1.0 ? 1.0 : c;
@@ -677,8 +683,8 @@ hipError_t launch_counts_to_lineint(const double* counts, float* out, int n_view
   size_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(counts_to_lineint_kernel, dim3((unsigned)blocks), dim3(256), 0, s, counts, out, n_views,
-                     nu * nv, photons_per_view);
+  hipLaunchKernelGGL(counts_to_lineint_kernel, dim3((unsigned)blocks), dim3(256), 0, s, counts, out, n_views, nu,
+                     nv, photons_per_view);
   return hipGetLastError();
 }
 

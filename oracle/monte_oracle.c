@@ -712,9 +712,10 @@ static void mc_items(void *vctx, long i0, long i1) {
       orc_stream rs;
       orc_stream_init(&rs, (uint32_t)n, (uint32_t)k, cfg->job, cfg->seed_lo, cfg->seed_hi);
       st.histories++;
-      /* aim at the centre of pixel n mod npix (CBCT_real325.cu:464-466) */
+      /* aim at the centre of pixel n mod npix (CBCT_real325.cu:464-466), in
+       * the order iv * nu + iu (u fastest) */
       const size_t pix = (size_t)(n % npix);
-      const int iu = (int)(pix / g->nv), iv = (int)(pix % g->nv);
+      const int iv = (int)(pix / g->nu), iu = (int)(pix % g->nu);
       const double yp = (iu + 0.5 - un0) / us, zp = (iv + 0.5 - vn0) / vs;
       double dl[3] = {g->sdd, yp, zp};
       double dn = 1.0 / sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);

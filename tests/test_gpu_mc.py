@@ -115,12 +115,12 @@ def test_counts_to_lineint(ctx):
     from monte_amd.mc import counts_to_lineint
 
     geom = G.small(n_views=2)
-    N = 61 * geom.nu * geom.nv + 1000  # uneven: the first 1000 pixels get 62 photons
+    N = 61 * geom.nu * geom.nv + 1000  # uneven: the first 1000 pixels in aim order get 62 photons
     rng = np.random.default_rng(5)
     counts = rng.integers(0, 70, size=geom.proj_shape).astype(np.float64)
     got = counts_to_lineint(torch.from_numpy(counts).cuda(), geom, N, ctx).cpu().numpy()
-    n0 = np.full(geom.nu * geom.nv, 61.0)
+    n0 = np.full(geom.nu * geom.nv, 61.0)  # aim order: iv * nu + iu
     n0[:1000] = 62.0
-    n0 = n0.reshape(geom.nu, geom.nv)[None]
+    n0 = n0.reshape(geom.nv, geom.nu).T[None]
     ref = np.log(n0) - np.log(np.clip(counts, 1.0, n0))
     assert np.allclose(got, ref, rtol=1e-6, atol=1e-6)
