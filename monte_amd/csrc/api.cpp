@@ -844,13 +844,16 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
   monte_tuning best = c->tuning;
   if (what & 1) {
     if (!g || !ph || !cfg) return fail(c, MONTE_ERR_INVALID, "MC tuning needs geometry, phantom and config");
-    // a short run: 2 views, up to 2^21 photons each, into scratch tallies
+    // a short run over all the job's views (units run views fastest, so the
+    // view count shapes the memory access pattern), about 2^23 photons in all
+    // and at least 2^16 per view, into scratch tallies
     monte_mc_cfg tc = *cfg;
-    tc.photons_per_view = std::min<uint64_t>(cfg->photons_per_view, 1u << 21);
     int vb = cfg->view_begin, ve = cfg->view_end;
     if (vb == 0 && ve == 0) ve = g->n_views;
+    const uint64_t per_view = std::max<uint64_t>(1u << 16, (1u << 23) / (uint64_t)std::max(ve - vb, 1));
+    tc.photons_per_view = std::min<uint64_t>(cfg->photons_per_view, per_view);
     tc.view_begin = vb;
-    tc.view_end = std::min(ve, vb + 2);
+    tc.view_end = ve;
     const size_t np = proj_elems(*g);
     double* tally = (double*)workspace(c, kSlotTuneA, 2 * np * sizeof(double));
     if (!tally) return fail(c, MONTE_ERR_NOMEM, "tuning tallies");

@@ -506,9 +506,11 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 // candidate columns of a range in parallel with the backprojector's own
 // column arithmetic and keep those whose taps fall in the strip.  Each kept
 // column is then walked along z by the whole wave: lane l takes the C
-// consecutive voxels of chunk l, from a z-fastest copy of the volume, with the
-// backprojector's per-voxel arithmetic (same column core, same 16-aligned
-// bases), so every weight equals the backprojector's bit for bit.
+// consecutive voxels of chunk l, from a z-fastest copy of the volume.  The
+// column core (u tap, fraction, distance weight) is the backprojector's own;
+// the v coordinate sv(z) = A + z D of each voxel is formed in 32.32 fixed
+// point from the column's FP64 A and D (integer row = high word, fraction =
+// low word), within 2^-32 rows of the backprojector's value.
 //
 // Accumulation is a plain read-add-write of LDS, without atomics: in one
 // instruction the active lanes' voxels are C apart along z, hence at least
@@ -582,10 +584,10 @@ constexpr int kPrefetchZ = 384;
 constexpr int kAhead = 4;
 // LDS floats after the strip that absorb the updates of lanes with nothing to
 // add (outside the v window or past the column end), so the updates need no
-// branches: two per lane, one row pitch and one cell of overlap.  (Branches
+// branches: two per lane, two row pitches and one cell of overlap.  (Branches
 // around the updates, or handing the upper-row update to the next lane by a
 // DPP shift, both measured slower.)
-constexpr int kTrash = 2 * 64 + kStripP + 2;
+constexpr int kTrash = 2 * 64 + 2 * kStripP + 2;
 
 template <int C>
 struct ColumnVals {
@@ -622,19 +624,38 @@ __device__ inline void load_column(ColumnVals<C>& cv, const float* __restrict__ 
     if (p < npass) load_chunk<C, MASK>(cv.v[p], col, p, x, y, g, lane);
 }
 
+// The v coordinate of a column in 32.32 fixed point: sv(z) = a + z d, and the
+// v window [lo, hi] (the backprojector's, inclusive).
+struct ColumnFx {
+  long long a, d, lo, hi;
+};
+
+__device__ inline ColumnFx column_fx(const ColumnCore& cc, const BpView& c, const BpGrid& g) {
+  ColumnFx f;
+  const double A = fma(cc.kv, g.gz0, c.sv0), D = cc.kv * g.gsz;
+  f.a = __double2ll_rn(fmin(fmax(A, -1.0e6), 1.0e6) * 4294967296.0);
+  f.d = __double2ll_rn(D * 4294967296.0);
+  f.lo = (long long)g.lo_v * 4294967296LL;
+  f.hi = (long long)g.hi_v * 4294967296LL;
+  return f;
+}
+
+// Row (floor) and fraction of a fixed-point coordinate.
+__device__ inline int fx_row(long long s) { return (int)(s >> 32); }
+__device__ inline float fx_frac(long long s) { return (float)(uint32_t)s * 2.3283064365386963e-10f; }  // 2^-32
+
 // The C voxel updates of one lane; lanes with `act` false (or nothing to add)
 // update their trash floats instead.
 template <int C>
 __device__ inline void pass_updates(float* __restrict__ strip, const float (&v)[C], const ColumnCore& cc,
-                                    const ChunkBase& cb, int joff, int z0, bool act, int cell0, int trash,
-                                    const BpGrid& g) {
+                                    long long s0, long long d, long long lo, long long hi, int z0, bool act,
+                                    int cell0, int trash, const BpGrid& g) {
 #pragma unroll
   for (int j = 0; j < C; ++j) {
-    int row;
-    float fv;
-    bool okv;
-    chunk_row(cc, cb, joff + j, row, fv, okv);
-    const bool live = act && okv && z0 + j < g.nz;
+    const long long sj = s0 + j * d;
+    const int row = fx_row(sj);
+    const float fv = fx_frac(sj);
+    const bool live = act && sj >= lo && sj <= hi && z0 + j < g.nz;
     const float sw = cc.w * v[j];
     const float bb = sw * fv, aa = sw - bb;
     const float a1 = aa * cc.fu, b1 = bb * cc.fu;
@@ -648,21 +669,57 @@ __device__ inline void pass_updates(float* __restrict__ strip, const float (&v)[
   }
 }
 
+// Both voxels of a 2-voxel chunk, when the v-slope is in (0, 1): the second
+// voxel's row is the first's or the one above, so the lane's four row updates
+// fold into three consecutive rows r, r + 1, r + 2, added in three rounds.  In
+// round k every lane updates its row r + k; the lanes' rows r strictly
+// increase, so no two lanes of a round touch the same cell.
+__device__ inline void pass_updates_pair(float* __restrict__ strip, const float (&v)[2], const ColumnCore& cc,
+                                         long long s0, long long d, long long lo, long long hi, int z0, int cell0,
+                                         int trash, const BpGrid& g) {
+  const long long s1 = s0 + d;
+  const int r0 = fx_row(s0), r1 = fx_row(s1);
+  const bool live0 = s0 >= lo && s0 <= hi && z0 < g.nz, live1 = s1 >= lo && s1 <= hi && z0 + 1 < g.nz;
+  const float sw0 = live0 ? cc.w * v[0] : 0.f, sw1 = live1 ? cc.w * v[1] : 0.f;
+  const float b0 = sw0 * fx_frac(s0), a0 = sw0 - b0;
+  const float b1 = sw1 * fx_frac(s1), a1 = sw1 - b1;
+  const bool same = r1 == r0;
+  const float q0 = a0 + (same ? a1 : 0.f);
+  const float q1 = b0 + (same ? b1 : a1);
+  const float q2 = same ? 0.f : b1;
+  const int i0 = (live0 || live1) ? cell0 + r0 * kStripP : trash;
+  const float t0 = q0 * cc.fu, t1 = q1 * cc.fu, t2 = q2 * cc.fu;
+  strip[i0] += q0 - t0;
+  strip[i0 + 1] += t0;
+  lds_order();
+  strip[i0 + kStripP] += q1 - t1;
+  strip[i0 + kStripP + 1] += t1;
+  lds_order();
+  strip[i0 + 2 * kStripP] += q2 - t2;
+  strip[i0 + 2 * kStripP + 1] += t2;
+  lds_order();
+}
+
 // One pass of a column: lane l takes the C voxels of chunk ch0 + l.  `cell0`
 // is the LDS index of (row 0, left tap); `trash` the lane's trash floats.
 template <int C>
-__device__ inline void project_pass(float* __restrict__ strip, const float (&v)[C], const BpView& c,
-                                    const ColumnCore& cc, int cell0, int trash, int ch0, int phases,
+__device__ inline void project_pass(float* __restrict__ strip, const float (&v)[C], const ColumnCore& cc,
+                                    const ColumnFx& fx, int cell0, int trash, int ch0, int phases,
                                     const BpGrid& g, int lane) {
   const int z0 = (ch0 + lane) * C;
-  const int zb = z0 & ~(kZBase - 1);
-  const ChunkBase cb = chunk_base(cc, c, base_z(g, zb), g);
-  const int joff = z0 - zb;
+  const long long s0 = fx.a + (long long)z0 * fx.d;
+  if constexpr (C == 2) {
+    if (phases == 1 && cc.dsv > 0.f && cc.dsv < 0.999f) {
+      pass_updates_pair(strip, v, cc, s0, fx.d, fx.lo, fx.hi, z0, cell0, trash, g);
+      return;
+    }
+  }
   if (phases == 1) {
-    pass_updates<C>(strip, v, cc, cb, joff, z0, true, cell0, trash, g);
+    pass_updates<C>(strip, v, cc, s0, fx.d, fx.lo, fx.hi, z0, true, cell0, trash, g);
   } else {
     const int ml = lane % phases;
-    for (int ph = 0; ph < phases; ++ph) pass_updates<C>(strip, v, cc, cb, joff, z0, ml == ph, cell0, trash, g);
+    for (int ph = 0; ph < phases; ++ph)
+      pass_updates<C>(strip, v, cc, s0, fx.d, fx.lo, fx.hi, z0, ml == ph, cell0, trash, g);
   }
 }
 
@@ -784,16 +841,17 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
     const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
     const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: -1 .. kStripW - 1
     const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + 1;
+    const ColumnFx fx = column_fx(cc, c, g);
 #pragma unroll
     for (int p = 0; p < NP; ++p)
-      if (p < npass) project_pass<C>(strip, vals.v[p], c, cc, cell0, trash + 2 * lane, p * 64, phases, g, lane);
+      if (p < npass) project_pass<C>(strip, vals.v[p], cc, fx, cell0, trash + 2 * lane, p * 64, phases, g, lane);
     if (npass > NP) {  // columns longer than the prefetch window load the rest in place
       const int x = bcast_i(cb.x, i), y = bcast_i(cb.y, i);
       const float* col = volT + ((size_t)y * g.nx + x) * g.nz;
       for (int p = NP; p < npass; ++p) {
         float v[C];
         load_chunk<C, MASK>(v, col, p, x, y, g, lane);
-        project_pass<C>(strip, v, c, cc, cell0, trash + 2 * lane, p * 64, phases, g, lane);
+        project_pass<C>(strip, v, cc, fx, cell0, trash + 2 * lane, p * 64, phases, g, lane);
       }
     }
   };

@@ -36,11 +36,14 @@ def _geoms():
     # rows, so the projector interleaves 2 and 3 lane phases per column
     thin = std.replace(nz=60, vz=0.02, n_views=12)
     thinner = std.replace(nz=90, vz=0.01, n_views=8)
+    # a v-slope of 0.8 rows per voxel, as in the 325^3 / 512^2 workload: two
+    # voxels per lane fold into three row updates
+    mid = std.replace(nz=45, vz=0.04, n_views=12)
     return {"std": std, "shepp": shepp, "bp3d20": compat, "bp3d20_mask": compat_mask, "thin": thin,
-            "thinner": thinner}
+            "thinner": thinner, "mid": mid}
 
 
-@pytest.mark.parametrize("name", ["std", "shepp", "bp3d20", "bp3d20_mask", "thin", "thinner"])
+@pytest.mark.parametrize("name", ["std", "shepp", "bp3d20", "bp3d20_mask", "thin", "thinner", "mid"])
 def test_dot_product_identity(ctx, name):
     from monte_amd.fdk import (fdk, fdk_adjoint, fdk_backproject, fdk_filter, fdk_filter_adjoint,
                                fdk_project)
@@ -65,7 +68,7 @@ def test_dot_product_identity(ctx, name):
     assert abs(f1 - f2) <= 1e-5 * abs(f1)
 
 
-@pytest.mark.parametrize("name", ["std", "thinner"])
+@pytest.mark.parametrize("name", ["std", "thinner", "mid"])
 def test_adjoint_matches_oracle(ctx, oracle, name):
     from monte_amd.fdk import fdk_adjoint
 
@@ -135,7 +138,7 @@ def test_autograd_batch_and_graph_of_ops(ctx):
     assert torch.allclose(x.grad, expect, rtol=1e-6, atol=1e-6 * expect.abs().max().item())
 
 
-@pytest.mark.parametrize("name", ["std", "thin", "bp3d20_mask"])
+@pytest.mark.parametrize("name", ["std", "thin", "mid", "bp3d20_mask"])
 def test_projector_agrees_across_lane_chunks(ctx, name):
     """Every voxels-per-lane variant of the projector is the same transpose:
     each passes the dot-product identity, and they agree with each other up
