@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=8, help="stacks in the differentiable-FDK step")
     ap.add_argument("--cpu-seconds", type=float, default=24.0, help="CPU-baseline budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="1: time candidate launch shapes first (the defaults are this workload's best)")
     return ap.parse_args()
 
 
@@ -179,9 +180,9 @@ def main():
 
 
 def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
-    """Oracle (FP64 C restatement) on the host cores, on a bounded sample: a
-    short probe sets the rate, then one run sized to about half the budget each
-    for MC and FDK."""
+    """Oracle (FP64 C restatement) on the host cores, on a bounded sample of
+    about half the budget each for MC and FDK: a probe run, then up to two runs
+    resized from the last one's time until it lands within 1.5x of the target."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -201,10 +202,15 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
         O.mc_run(og, oph, tb, n, spectrum_cdf=cdf, coherent_mode=1, views=(0, 2), nthreads=nthreads)
         return time.perf_counter() - t0
 
-    probe = 200_000
-    rate = 2 * probe / max(mc_time(probe), 1e-3)
-    n = max(probe, int(rate * target / 2))
+    # size the sample from a probe, then correct once if the probe (which pays
+    # the fixed per-run costs) mispredicted by more than a factor of 1.5
+    n = 2_000_000
     dt = mc_time(n)
+    for _ in range(2):
+        if target / 1.5 <= dt <= target * 1.5:
+            break
+        n = max(n, int(n * target / max(dt, 1e-3)))
+        dt = mc_time(n)
     mc = {"value": 2 * n / dt, "unit": "histories/s", "cores": nthreads, "kind": "port",
           "sample": f"oracle MC (FP64, same physics and Philox streams) on 2 views x {n} photons of the "
                     f"workload, {dt:.1f} s", "gpu_over_cpu": gpu_hps / (2 * n / dt)}
@@ -216,9 +222,13 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
         O.fdk_std(g, proj, nthreads)
         return time.perf_counter() - t0
 
-    per_view = fdk_time(1)
-    nv = int(min(geom.n_views, max(2, target / max(per_view, 1e-3))))
+    nv = 2
     dt = fdk_time(nv)
+    for _ in range(2):
+        if target / 1.5 <= dt <= target * 1.5 or nv >= geom.n_views:
+            break
+        nv = int(min(geom.n_views, max(nv + 1, nv * target / max(dt, 1e-3))))
+        dt = fdk_time(nv)
     ups = nv * geom.nx * geom.ny * geom.nz / dt
     fdk = {"value": ups / 1e9, "unit": "GUPS", "cores": nthreads, "kind": "port",
            "sample": f"oracle FDK (FP64: weight, direct-convolution filter, backprojection) of {nv} views "

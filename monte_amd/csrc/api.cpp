@@ -392,8 +392,8 @@ monte_tuning default_tuning() {
   t.mc_batch = 16;
   t.mc_blocks_per_cu = 4;
   t.bp_zchunk = 8;
-  t.bp_tile_x = 16;
-  t.bp_tile_y = 16;
+  t.bp_tile_x = 8;   // fastest at 325^3 / 512^2 / 360 views
+  t.bp_tile_y = 32;
   t.bp_view_chunk = 0;
   t.pj_zchunk = 0;
   return t;
@@ -889,7 +889,7 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
     const float* taps;
     const BpView* views;
     if ((rc = ensure_constants(c, tg, &taps, &views))) return rc;
-    const int nviews = std::min(g->n_views, 32);
+    const int nviews = std::min(g->n_views, 360);  // L2 reuse across views shapes the best tile
     float* pad = (float*)workspace(c, kSlotFwd, padded_elems(tg) * sizeof(float));
     float* vol = (float*)workspace(c, kSlotTuneB, vol_elems(tg) * sizeof(float));
     if (!pad || !vol) return fail(c, MONTE_ERR_NOMEM, "tuning buffers");

@@ -50,6 +50,16 @@ class McSettings:
         return cfg, keep
 
 
+def photons_per_pixel(geom: Geometry, photons_per_view: int) -> np.ndarray:
+    """Photons aimed at each pixel [nu][nv]: photon n aims at pixel n mod
+    (nu nv) in the order iv * nu + iu, so the first N mod (nu nv) pixels of that
+    order get one more than the rest."""
+    npix = geom.nu * geom.nv
+    base, rem = divmod(int(photons_per_view), npix)
+    order = np.arange(npix, dtype=np.int64).reshape(geom.nv, geom.nu).T  # [iu][iv] -> iv * nu + iu
+    return (base + (order < rem)).astype(np.float64)
+
+
 def device_labels(ph: LabelPhantom, device) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(ph.labels)).to(device)
 

@@ -112,7 +112,7 @@ def test_counts_to_lineint(ctx):
     import torch
 
     from monte_amd import geometry as G
-    from monte_amd.mc import counts_to_lineint
+    from monte_amd.mc import counts_to_lineint, photons_per_pixel
 
     geom = G.small(n_views=2)
     N = 61 * geom.nu * geom.nv + 1000  # uneven: the first 1000 pixels in aim order get 62 photons
@@ -122,5 +122,6 @@ def test_counts_to_lineint(ctx):
     n0 = np.full(geom.nu * geom.nv, 61.0)  # aim order: iv * nu + iu
     n0[:1000] = 62.0
     n0 = n0.reshape(geom.nv, geom.nu).T[None]
+    assert np.array_equal(n0[0], photons_per_pixel(geom, N))
     ref = np.log(n0) - np.log(np.clip(counts, 1.0, n0))
     assert np.allclose(got, ref, rtol=1e-6, atol=1e-6)
