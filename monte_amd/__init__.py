@@ -5,6 +5,12 @@ The compute lives in libmonte.so (hand-written HIP kernels behind the C ABI in
 include/monte.h); this package is its Python face over torch device tensors.
 Importing it without the built library raises: there is no CPU path.
 """
+# torch first: its wheel ships its own libamdhip64.so.7, and libmonte.so must
+# bind to that same HIP runtime (same soname, so the loader reuses whichever
+# was loaded first).  Loading libmonte.so first would pull in a second HIP
+# runtime beside torch's, and streams and pointers would not carry across.
+import torch  # noqa: F401,E402
+
 from . import _lib
 from ._lib import MonteError
 

@@ -176,3 +176,17 @@ def test_create_without_gpu_reports_error():
     h = C.c_void_p()
     assert _lib.lib().monte_create(C.byref(h), 0) < 0
     assert not h.value
+
+
+def test_one_hip_runtime_with_torch():
+    """Importing monte_amd (even before torch) leaves one HIP runtime in the
+    process: libmonte.so binds to the libamdhip64 that torch loaded."""
+    import subprocess
+    import sys
+
+    code = ("import re, monte_amd, torch\n"
+            "maps = open('/proc/self/maps').read()\n"
+            "print(len(set(re.findall(r'\\S*libamdhip64\\S*', maps))))\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True,
+                         cwd=__import__("os").path.dirname(__import__("os").path.dirname(__file__)))
+    assert out.stdout.strip().splitlines()[-1] == "1", out.stdout + out.stderr
