@@ -181,6 +181,8 @@ int run_project_padded(monte_ctx* c, const monte_geom& g, const float* vol, floa
   float* volT = (float*)workspace(c, kSlotVolT, vol_elems(g) * sizeof(float));
   if (!volT) return fail(c, MONTE_ERR_NOMEM, "transposed volume allocation failed");
   MONTE_HIP(c, launch_transpose_zx(vol, volT, g.nx, g.ny, g.nz, c->stream));
+  // strip edge columns are added atomically onto zero
+  MONTE_HIP(c, launch_fill(pad, 0.f, padded_elems(g), c->stream));
   std::vector<BpView> host_views;
   compute_bp_views(g, host_views);
   const int chunk = c->tuning.bp_view_chunk > 0 ? c->tuning.bp_view_chunk : g.n_views;

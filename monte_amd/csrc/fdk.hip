@@ -499,7 +499,8 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 // The matched projector, written as a gather so no atomics are needed.
 // One wave owns a strip of kStripW padded detector columns (all rows) of one
 // view, held in its own LDS.  It enumerates the voxel columns whose left tap
-// can land in the strip: for each slab of the volume across the view's rays
+// lands in the strip (each voxel column of the view belongs to exactly one
+// strip; its right tap may spill into the next strip's first column): for each slab of the volume across the view's rays
 // (x-slabs when |cos b| >= |sin b|, y-slabs otherwise) the projective map
 // su(X,Y) is inverted at the strip's two edges, giving a range of columns on
 // that slab (widened by one voxel each side).  The lanes evaluate the
@@ -520,8 +521,8 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 // row is updated by a second read-add-write after the first completes, and a
 // wave's LDS operations execute in program order, so updates of a cell by
 // neighbouring lanes, later voxels and later columns all see each other.
-// Strip columns 0 and kStripW + 1 are guards that absorb taps landing just
-// outside the strip.  The row pitch sets how the lanes' strided rows fall on
+// LDS column 0 is unused and column kStripW + 1 collects the right-tap
+// spill into the next strip.  The row pitch sets how the lanes' strided rows fall on
 // LDS banks: at 325^3 / 512^2 a pitch of 7 ran 20% faster than 9 and 30%
 // faster than the even pitches 6 and 10, and a 5-column strip balances the
 // per-strip enumeration against the waves a CU can hold.
@@ -768,7 +769,7 @@ struct Enumerator {
       const int x = min(xo, g.x1 - 1), y = min(yo, g.ny - 1);
       const double X = voxel_x(g, x), Y = voxel_y(g, y);
       const ColumnCore mine = column_core(c, X, Y, (float)X, (float)Y, g);
-      const bool keep = sl < n_slabs && off <= b && mine.ok && mine.iu >= u_lo - 1 && mine.iu <= u_hi;
+      const bool keep = sl < n_slabs && off <= b && mine.ok && mine.iu >= u_lo && mine.iu <= u_hi;
       const uint64_t m = __ballot(keep);
       const int cnt = (int)__popcll(m);
       if (cb.n + cnt > 64) return true;  // full: this round is taken again by the next fill
@@ -813,8 +814,8 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
 
   const BpView& c = views[k];
   const bool x_slabs = fabs(c.cx) >= fabs(c.cy);
-  // columns whose left tap iu satisfies u_lo - 1 <= iu <= u_hi, i.e. su in [u_lo - 1, u_hi + 1)
-  const double s_lo = (double)(u_lo - 1) - 0.5, s_hi = (double)(u_hi + 1) + 0.5;
+  // columns whose left tap iu satisfies u_lo <= iu <= u_hi, i.e. su in [u_lo, u_hi + 1)
+  const double s_lo = (double)u_lo - 0.5, s_hi = (double)(u_hi + 1) + 0.5;
   const int n_slabs = x_slabs ? (g.x1 - g.x0) : g.ny;
   const int nch = (g.nz + C - 1) / C;
   const int npass = (nch + 63) / 64;
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
     cc.dsv = bcast_f(cb.dsv, i);
     const float step = (float)C * fabsf(cc.dsv);
     const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
-    const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: -1 .. kStripW - 1
+    const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: 0 .. kStripW - 1
     const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + 1;
     const ColumnFx fx = column_fx(cc, c, g);
 #pragma unroll
@@ -873,12 +874,27 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
     }
   }
   __syncthreads();
+  // Strip column t (LDS column t + 1) is padded column pu0 + t.  Columns 1 ..
+  // kStripW - 1 belong to this strip alone; column 0 also takes the previous
+  // strip's spill, and the spill column kStripW is the next strip's column 0.
+  // Those two are added atomically into the zeroed output: two addends onto
+  // zero, so the sum is the same in either order.
   float* out = filt + (size_t)k * g.rows * g.pitch;
-  for (int i = lane; i < g.rows * kStripW; i += 64) {
-    const int r = i / kStripW, t = i - r * kStripW;
+  constexpr int kOut = kStripW + 1;
+  for (int i = lane; i < g.rows * kOut; i += 64) {
+    const int r = i / kOut, t = i - r * kOut;
     const int lr = r - lds_r0;
-    if (pu0 + t < g.pitch)
-      out[(size_t)r * g.pitch + pu0 + t] = (lr >= 0 && lr < lds_rows) ? strip[lr * kStripP + t + 1] : 0.f;
+    if (pu0 + t >= g.pitch) continue;
+    float* o = out + (size_t)r * g.pitch + pu0 + t;
+    const bool held = lr >= 0 && lr < lds_rows;
+    if (t == 0 || t == kStripW) {
+      if (held) {
+        const float val = strip[lr * kStripP + t + 1];
+        if (val != 0.f) unsafeAtomicAdd(o, val);
+      }
+    } else {
+      *o = held ? strip[lr * kStripP + t + 1] : 0.f;
+    }
   }
 }
 
