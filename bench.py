@@ -80,9 +80,9 @@ def main():
         stacks.append((np.log(1e5) - torch.log(torch.clamp(counts, min=1.0))).float())
     stacks = torch.stack(stacks).contiguous()
     del L
-    tuning = {}
     if a.autotune:
-        tuning = autotune(ph, geom, settings, what=3, labels_dev=labels, ctx=ctx)
+        autotune(ph, geom, settings, what=3, labels_dev=labels, ctx=ctx)
+    tuning = ctx.tuning
     torch.cuda.synchronize()
 
     def step():
@@ -164,8 +164,9 @@ def main():
         "roofline": {
             "bound": "valu", "kernel": "mc_transport_kernel",
             "achieved": w_mc * hist / mc_k / 1e12, "peak": PEAK_LANE_OPS / 1e12, "unit": "Tlane-op/s",
-            "frac": w_mc * hist / mc_k / PEAK_LANE_OPS, "traffic": None,
+            "frac": w_mc * hist / mc_k / PEAK_LANE_OPS, "traffic": profiled_traffic("mc_transport_kernel", a, N),
             "work_per_unit": "W = 40 S + 150 C + 60 lane-ops per history (S steps, C interactions)",
+            "kernel_ms": mc_k * 1e3,
         },
         "roofline_fdk_backproject": {
             "bound": "valu", "achieved": BP_OPS * updates / fdk_s / 1e12, "peak": PEAK_LANE_OPS / 1e12,
@@ -177,6 +178,18 @@ def main():
     if a.cpu_seconds > 0:
         out["cpu_baseline"], out["cpu_baseline_fdk"] = cpu_baselines(a, geom, ph, spec, hist / mc_s, updates / fdk_s)
     print(json.dumps(out))
+
+
+def profiled_traffic(kernel, a, photons):
+    """Memory traffic per launch (bytes) of `kernel` on this workload, from the
+    FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/ (scaled by a
+    copy of known size); None for any other workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not (os.path.exists(path) and a.views == 360 and photons == int(1e8)):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(kernel)
+    return rec["bytes_per_launch"] if rec else None
 
 
 def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
