@@ -265,9 +265,10 @@ __device__ inline int label_at(const Photon& p, float t, const McJob& J, float& 
   const int ix = min(max((int)floorf((qx - J.bx0) * J.inv_vx), 0), J.nx - 1);
   const int iy = min(max((int)floorf((qy - J.by0) * J.inv_vy), 0), J.ny - 1);
   const int iz = min(max((int)floorf((qz - J.bz0) * J.inv_vz), 0), J.nz - 1);
-  // 32-bit voxel index (checked on the host) and a global-address-space load
+  // 32-bit voxel index (checked on the host) and a global-address-space load;
+  // labels past the table are clamped so no table read leaves it
   const uint32_t i = ((uint32_t)iz * (uint32_t)J.ny + (uint32_t)iy) * (uint32_t)J.nx + (uint32_t)ix;
-  return ((const __attribute__((address_space(1))) uint8_t*)J.labels)[i];
+  return min((int)((const __attribute__((address_space(1))) uint8_t*)J.labels)[i], kMaxLabels - 1);
 }
 
 // Wave-uniform statistics (scalar registers), flushed per job.
