@@ -296,6 +296,8 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
   J.by0 = (float)(ph->oy - 0.5 * ph->ny * ph->vy); J.by1 = (float)(ph->oy + 0.5 * ph->ny * ph->vy);
   J.bz0 = (float)(ph->oz - 0.5 * ph->nz * ph->vz); J.bz1 = (float)(ph->oz + 0.5 * ph->nz * ph->vz);
   J.inv_vx = (float)(1.0 / ph->vx); J.inv_vy = (float)(1.0 / ph->vy); J.inv_vz = (float)(1.0 / ph->vz);
+  J.ix0 = (float)(-(double)J.bx0 / ph->vx); J.iy0 = (float)(-(double)J.by0 / ph->vy);
+  J.iz0 = (float)(-(double)J.bz0 / ph->vz);
   J.sod = (float)g->sod; J.sdd = (float)g->sdd;
   if (g->preset == MONTE_PRESET_BP3D20) {
     // CBCT_real325.cu:565-566: index -(int)(10 d - 162.5), i.e. u_idx = nu/2 - y'/du

@@ -98,6 +98,7 @@ struct McJob {
   float vx, vy, vz;
   float bx0, by0, bz0, bx1, by1, bz1;  // phantom box
   float inv_vx, inv_vy, inv_vz;
+  float ix0, iy0, iz0;           // -b0 / v: voxel index = q inv_v + i0
   float sod, sdd;
   float un0, us, vn0, vs;        // u_idx = un0 + us * y', v_idx = vn0 + vs * z'
   float u_step, v_step;          // 1/us, 1/vs (signed pitch)

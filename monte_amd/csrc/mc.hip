@@ -50,7 +50,8 @@ struct Stream {
   int left;
 };
 
-__device__ inline float u01(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-08f; }  // 2^-24
+// ((x >> 8) + 1/2) 2^-24 in one rounding
+__device__ inline float u01(uint32_t x) { return fmaf((float)(x >> 8), 5.9604644775390625e-08f, 2.98023223876953125e-08f); }
 
 __device__ inline float next_u01(Stream& s, uint32_t job, uint32_t k0, uint32_t k1) {
   if (s.left == 0) {
@@ -127,7 +128,7 @@ __device__ inline int ebin_of(float e) {
 // Slab test along one axis, folded into [lo, hi]; false if the ray misses.
 __device__ inline bool slab(float p, float d, float b0, float b1, float& lo, float& hi) {
   if (d != 0.f) {
-    const float inv = 1.0f / d;
+    const float inv = __builtin_amdgcn_rcpf(d);  // 1 ulp: the box clip has voxel-scale slack
     float ta = (b0 - p) * inv, tb = (b1 - p) * inv;
     lo = fmaxf(lo, fminf(ta, tb));
     hi = fminf(hi, fmaxf(ta, tb));
@@ -203,7 +204,7 @@ __device__ inline int tally(const Photon& p, const McJob& J) {
   const float xq = p.px * cx + p.py * cy, yq = -p.px * cy + p.py * cx;
   const float ddx = p.dx * cx + p.dy * cy, ddy = -p.dx * cy + p.dy * cx;
   if (!(ddx > 0.f)) return kTallyMiss;
-  const float tt = ((J.sdd - J.sod) - xq) / ddx;
+  const float tt = ((J.sdd - J.sod) - xq) * __builtin_amdgcn_rcpf(ddx);
   const float yh = fmaf(tt, ddy, yq), zh = fmaf(tt, p.dz, p.pz);
   const float ui = fmaf(J.us, yh, J.un0), vi = fmaf(J.vs, zh, J.vn0);
   if (!(ui >= 0.f && ui < (float)J.nu && vi >= 0.f && vi < (float)J.nv)) return kTallyMiss;
@@ -262,9 +263,11 @@ __device__ inline float mu_at(const Photon& p, int lab, const McJob& J) {
 
 __device__ inline int label_at(const Photon& p, float t, const McJob& J, float& qx, float& qy, float& qz) {
   qx = fmaf(t, p.dx, p.px); qy = fmaf(t, p.dy, p.py); qz = fmaf(t, p.dz, p.pz);
-  const int ix = min(max((int)floorf((qx - J.bx0) * J.inv_vx), 0), J.nx - 1);
-  const int iy = min(max((int)floorf((qy - J.by0) * J.inv_vy), 0), J.ny - 1);
-  const int iz = min(max((int)floorf((qz - J.bz0) * J.inv_vz), 0), J.nz - 1);
+  // voxel index = (q - b0) / v as one FMA; truncation equals floor wherever
+  // the clamp to [0, n - 1] does not already decide it
+  const int ix = min(max((int)fmaf(qx, J.inv_vx, J.ix0), 0), J.nx - 1);
+  const int iy = min(max((int)fmaf(qy, J.inv_vy, J.iy0), 0), J.ny - 1);
+  const int iz = min(max((int)fmaf(qz, J.inv_vz, J.iz0), 0), J.nz - 1);
   // 32-bit voxel index (checked on the host) and a global-address-space load;
   // labels past the table are clamped so no table read leaves it
   const uint32_t i = ((uint32_t)iz * (uint32_t)J.ny + (uint32_t)iy) * (uint32_t)J.nx + (uint32_t)ix;
