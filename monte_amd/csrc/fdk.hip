@@ -34,181 +34,186 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------- filter
-// Convolution core shared by the forward and transposed filters.  LDS holds
-// A[TV][nu+1] (input rows, +1 pad against bank conflicts) and the taps
-// H[2nu-1] (centre nu-1).  Wave w computes rows [w*RW, (w+1)*RW), lane l the
-// columns b = l + 64 m, m < NB:  out[j][b] = sum_c A[j][c] H[c - b + nu - 1].
-template <int NB, int RW>
-__device__ inline void conv_rows(const float* __restrict__ A, const float* __restrict__ H, int nu, int lda,
-                                 float (&acc)[RW][NB]) {
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < RW; ++q)
-#pragma unroll
-    for (int m = 0; m < NB; ++m) acc[q][m] = 0.f;
-  int hb[NB];
-#pragma unroll
-  for (int m = 0; m < NB; ++m) {
-    int b = lane + 64 * m;
-    if (b > nu - 1) b = nu - 1;  // lanes past the row compute a discarded copy
-    hb[m] = nu - 1 - b;
-  }
-  const float* Arow = A + (w * RW) * lda;
-  for (int c = 0; c < nu; ++c) {
-    float a[RW];
-#pragma unroll
-    for (int q = 0; q < RW; ++q) a[q] = Arow[q * lda + c];
-#pragma unroll
-    for (int m = 0; m < NB; ++m) {
-      const float h = H[c + hb[m]];
-#pragma unroll
-      for (int q = 0; q < RW; ++q) acc[q][m] = fmaf(a[q], h, acc[q][m]);
-    }
-  }
-}
-
 __device__ inline float cos_weight(const FilterParams& fp, int r, int c) {
   const float a = fmaf(fp.b_u, (float)r, fp.a_u);
   const float b = fmaf(fp.b_v, (float)c, fp.a_v);
   return fp.D / sqrtf(fmaf(fp.D, fp.D, fmaf(a, a, b * b)));
 }
 
-// proj [view][nu][nv]  ->  filt_pad [view][rows][pitch] (interior at +kPadLo)
-template <int NB, int TV>
-__global__ __launch_bounds__(256) void filter_fwd_kernel(const float* __restrict__ proj, float* __restrict__ out,
-                                                         const float* __restrict__ taps, FilterParams fp) {
-  constexpr int RW = TV / 4;
-  extern __shared__ float sm[];
-  const int nu = fp.nu, nv = fp.nv;
-  const int lda = nu + 1;
-  float* A = sm;
-  float* H = sm + TV * lda;
-  const int view = blockIdx.y;
-  const int iv0 = blockIdx.x * TV;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < 2 * nu - 1; i += 256) H[i] = taps[i];
-  const float* src = proj + (size_t)view * nu * nv;
-  for (int i = tid; i < nu * TV; i += 256) {
-    const int c = i / TV, j = i - c * TV;  // detector row c (u), column iv0 + j (v)
-    const int iv = iv0 + j;
-    float v = 0.f;
-    if (iv < nv) v = src[(size_t)c * nv + iv] * cos_weight(fp, c, iv);
-    A[j * lda + c] = v;
+// The ramp filter of one view is a product with a Toeplitz matrix:
+//   out[v][b] = cs sum_c A[v][c] H[c][b],  H[c][b] = h[c - b + nu - 1],
+// with A the cosine-weighted projection (forward) or the filtered stack
+// (transpose; h is symmetric, so the same H serves).  Each block computes a
+// 64 (v) x 128 (b) tile with v_mfma_f32_32x32x2_f32: wave w owns rows
+// 32 (w & 1) and the two 32-column tiles at 64 (w >> 1).  A is staged in LDS
+// 32 columns of c at a time as As[c][v]; H is read straight from the taps in
+// LDS, shifted by kFbn so every index of the tile stays inside the array.
+// Operand layouts (one wave, 32 x 32 x 2): A from lane l is A[l % 32][l / 32],
+// B from lane l is B[l / 32][l % 32], and accumulator i of lane l is
+// D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
+constexpr int kFbm = 64, kFbn = 128, kFkc = 32;
+constexpr int kFas = kFbm + 1;  // LDS row pitch of As
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Taps into LDS: Ht[i + kFbn] = h[i], zero elsewhere (2 nu + kFkc + 2 kFbn floats).
+__device__ inline void stage_taps(float* __restrict__ Ht, const float* __restrict__ taps, int nu) {
+  const int n = 2 * nu + kFkc + 2 * kFbn;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int j = i - kFbn;
+    Ht[i] = (j >= 0 && j < 2 * nu - 1) ? taps[j] : 0.f;
   }
-  __syncthreads();
-  float acc[RW][NB];
-  conv_rows<NB, RW>(A, H, nu, lda, acc);
-  const int lane = tid & 63, w = tid >> 6;
+}
+
+// acc[t] += A[rows][c0 .. c0 + kFkc) x H[c0 .. c0 + kFkc)[cols of tile t]
+__device__ inline void mfma_chunk(const float* __restrict__ As, const float* __restrict__ Ht, int c0, int b0, int nu,
+                                  floatx16 (&acc)[2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = (w & 1) * 32 + (lane & 31), kl = lane >> 5;
+  const int col = b0 + (w >> 1) * 64 + (lane & 31);
+  // H index of (c0 + kk + kl, col + 32 t) is c - b + nu - 1 + kFbn
+  const int h0 = c0 + kl - col + nu - 1 + kFbn;
+#pragma unroll
+  for (int kk = 0; kk < kFkc; kk += 2) {
+    const float a = As[(kk + kl) * kFas + r];
+    const float bv0 = Ht[h0 + kk], bv1 = Ht[h0 + kk - 32];
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv1, acc[1], 0, 0, 0);
+  }
+}
+
+// proj [view][nu][nv] -> filt_pad [view][rows][pitch] (interior at +kPadLo).
+// Thread (tc, tv) stages A[c0 + tc + 4 j][v0 + tv], j < 8; the next chunk's
+// loads are issued before the current chunk's products, so their latency
+// hides behind the MFMAs.
+__global__ __launch_bounds__(256) void filter_fwd_mfma_kernel(const float* __restrict__ proj,
+                                                              float* __restrict__ out,
+                                                              const float* __restrict__ taps, FilterParams fp) {
+  extern __shared__ float sm[];
+  float* As = sm;  // [kFkc][kFas]
+  float* Ht = sm + kFkc * kFas;
+  const int nu = fp.nu, nv = fp.nv;
+  const int v0 = blockIdx.x * kFbm, b0 = blockIdx.y * kFbn, view = blockIdx.z;
+  stage_taps(Ht, taps, nu);
+  const float* src = proj + (size_t)view * nu * nv;
+  const int tv = threadIdx.x & 63, tc = threadIdx.x >> 6;
+  const int v = v0 + tv, vc = v < nv ? v : nv - 1;
+  float pre[kFkc / 4];
+  auto load = [&](int c0) {
+#pragma unroll
+    for (int j = 0; j < kFkc / 4; ++j) {
+      const int c = c0 + tc + 4 * j;
+      pre[j] = src[(size_t)(c < nu ? c : nu - 1) * nv + vc];
+    }
+  };
+  load(0);
+  floatx16 acc[2] = {};
+  for (int c0 = 0; c0 < nu; c0 += kFkc) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kFkc / 4; ++j) {
+      const int kc = tc + 4 * j, c = c0 + kc;
+      As[kc * kFas + tv] = (c < nu && v < nv) ? pre[j] * cos_weight(fp, c, v) : 0.f;
+    }
+    __syncthreads();
+    if (c0 + kFkc < nu) load(c0 + kFkc);
+    mfma_chunk(As, Ht, c0, b0, nu, acc);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float* dst = out + (size_t)view * fp.rows * fp.pitch;
 #pragma unroll
-  for (int q = 0; q < RW; ++q) {
-    const int iv = iv0 + w * RW + q;
-    if (iv >= nv) continue;
+  for (int t = 0; t < 2; ++t) {
+    const int b = b0 + (w >> 1) * 64 + 32 * t + (lane & 31);
 #pragma unroll
-    for (int m = 0; m < NB; ++m) {
-      const int b = lane + 64 * m;
-      if (b < nu) dst[(size_t)(iv + kPadLo) * fp.pitch + b + kPadLo] = fp.cs * acc[q][m];
+    for (int i = 0; i < 16; ++i) {
+      const int vo = v0 + (w & 1) * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i & 3);
+      if (vo < nv && b < nu) dst[(size_t)(vo + kPadLo) * fp.pitch + b + kPadLo] = fp.cs * acc[t][i];
     }
   }
 }
 
-// filt_pad [view][rows][pitch] (interior)  ->  proj [view][nu][nv], weighted:
-// the transpose of filter_fwd (the taps are symmetric).
-template <int NB, int TV>
-__global__ __launch_bounds__(256) void filter_T_kernel(const float* __restrict__ in, float* __restrict__ proj,
-                                                       const float* __restrict__ taps, FilterParams fp) {
-  constexpr int RW = TV / 4;
+// filt_pad [view][rows][pitch] (interior) -> proj [view][nu][nv], weighted: the
+// transpose of the forward filter.  Thread (tk, tr) stages A[v0 + tr + 8 j][c0 + tk],
+// j < 8, pipelined like the forward kernel; results go through LDS so the
+// [u][v] output is written in runs along v.
+__global__ __launch_bounds__(256) void filter_T_mfma_kernel(const float* __restrict__ in, float* __restrict__ proj,
+                                                            const float* __restrict__ taps, FilterParams fp) {
   extern __shared__ float sm[];
+  float* As = sm;
+  float* Ht = sm + kFkc * kFas;
   const int nu = fp.nu, nv = fp.nv;
-  const int lda = nu + 1;
-  float* A = sm;
-  float* H = sm + TV * lda;
-  const int view = blockIdx.y;
-  const int iv0 = blockIdx.x * TV;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < 2 * nu - 1; i += 256) H[i] = taps[i];
+  const int v0 = blockIdx.x * kFbm, b0 = blockIdx.y * kFbn, view = blockIdx.z;
+  stage_taps(Ht, taps, nu);
   const float* src = in + (size_t)view * fp.rows * fp.pitch;
-  for (int i = tid; i < nu * TV; i += 256) {
-    const int j = i / nu, b = i - j * nu;
-    const int iv = iv0 + j;
-    A[j * lda + b] = iv < nv ? src[(size_t)(iv + kPadLo) * fp.pitch + b + kPadLo] : 0.f;
-  }
-  __syncthreads();
-  float acc[RW][NB];
-  conv_rows<NB, RW>(A, H, nu, lda, acc);
-  __syncthreads();
-  // stage results in LDS as [c][j] so the [u][v] output is written in runs along v
-  const int lane = tid & 63, w = tid >> 6;
+  const int tk = threadIdx.x & 31, tr = threadIdx.x >> 5;
+  float pre[kFbm / 8];
+  auto load = [&](int c0) {
+    const int c = c0 + tk, cc = c < nu ? c : nu - 1;
 #pragma unroll
-  for (int q = 0; q < RW; ++q)
-#pragma unroll
-    for (int m = 0; m < NB; ++m) {
-      const int c = lane + 64 * m;
-      if (c < nu) A[c * TV + w * RW + q] = acc[q][m];
+    for (int j = 0; j < kFbm / 8; ++j) {
+      const int v = v0 + tr + 8 * j;
+      pre[j] = src[(size_t)((v < nv ? v : nv - 1) + kPadLo) * fp.pitch + cc + kPadLo];
     }
+  };
+  load(0);
+  floatx16 acc[2] = {};
+  for (int c0 = 0; c0 < nu; c0 += kFkc) {
+    __syncthreads();
+    const bool cin = c0 + tk < nu;
+#pragma unroll
+    for (int j = 0; j < kFbm / 8; ++j) {
+      const int vl = tr + 8 * j;
+      As[tk * kFas + vl] = (cin && v0 + vl < nv) ? pre[j] : 0.f;
+    }
+    __syncthreads();
+    if (c0 + kFkc < nu) load(c0 + kFkc);
+    mfma_chunk(As, Ht, c0, b0, nu, acc);
+  }
+  // stage D as Ds[b_local][v_local] over the same LDS, then write proj[b][v] along v
+  __syncthreads();
+  float* Ds = sm;  // [kFbn][kFas]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int bl = (w >> 1) * 64 + 32 * t + (lane & 31);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int vl = (w & 1) * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i & 3);
+      Ds[bl * kFas + vl] = acc[t][i];
+    }
+  }
   __syncthreads();
   float* dst = proj + (size_t)view * nu * nv;
-  for (int i = tid; i < nu * TV; i += 256) {
-    const int c = i / TV, j = i - c * TV;
-    const int iv = iv0 + j;
-    if (iv < nv) dst[(size_t)c * nv + iv] = fp.cs * A[c * TV + j] * cos_weight(fp, c, iv);
+  for (int i = threadIdx.x; i < kFbn * kFbm; i += 256) {
+    const int bl = i / kFbm, vl = i - bl * kFbm;
+    const int b = b0 + bl, v = v0 + vl;
+    if (b < nu && v < nv) dst[(size_t)b * nv + v] = fp.cs * Ds[bl * kFas + vl] * cos_weight(fp, b, v);
   }
 }
 
-template <template <int, int> class K>
-struct FilterLaunch;
-
-static int filter_nb(int nu) {
-  int nb = (nu + 63) / 64;
-  if (nb <= 1) return 1;
-  if (nb <= 2) return 2;
-  if (nb <= 4) return 4;
-  if (nb <= 8) return 8;
-  if (nb <= 16) return 16;
-  return -1;
+// LDS bytes: As and the taps; the transposed filter also stages its 128 x 64
+// result over the same memory.
+static size_t filter_mfma_lds(int nu, bool transposed) {
+  const size_t stage = (size_t)kFkc * kFas + 2 * (size_t)nu + kFkc + 2 * kFbn;
+  const size_t out = transposed ? (size_t)kFbn * kFas : 0;
+  return sizeof(float) * (stage > out ? stage : out);
 }
-
-#define MONTE_FILTER_DISPATCH(KERNEL, NBV, TVV, ...)                                                   \
-  do {                                                                                                \
-    if (TVV == 16) {                                                                                  \
-      switch (NBV) {                                                                                  \
-        case 1: hipLaunchKernelGGL((KERNEL<1, 16>), __VA_ARGS__); break;                              \
-        case 2: hipLaunchKernelGGL((KERNEL<2, 16>), __VA_ARGS__); break;                              \
-        case 4: hipLaunchKernelGGL((KERNEL<4, 16>), __VA_ARGS__); break;                              \
-        case 8: hipLaunchKernelGGL((KERNEL<8, 16>), __VA_ARGS__); break;                              \
-        default: return hipErrorInvalidValue;                                                         \
-      }                                                                                               \
-    } else {                                                                                          \
-      switch (NBV) {                                                                                  \
-        case 8: hipLaunchKernelGGL((KERNEL<8, 8>), __VA_ARGS__); break;                               \
-        case 16: hipLaunchKernelGGL((KERNEL<16, 8>), __VA_ARGS__); break;                             \
-        default: return hipErrorInvalidValue;                                                         \
-      }                                                                                               \
-    }                                                                                                 \
-  } while (0)
 
 hipError_t launch_fdk_filter(const float* proj, float* filt_pad, const float* taps, const FilterParams& fp,
                              hipStream_t s) {
-  const int nb = filter_nb(fp.nu);
-  if (nb < 0) return hipErrorInvalidValue;
-  const int tv = fp.nu <= 512 ? 16 : 8;
-  const size_t lds = sizeof(float) * ((size_t)tv * (fp.nu + 1) + 2 * fp.nu - 1);
-  dim3 grid((fp.nv + tv - 1) / tv, fp.n_views);
-  MONTE_FILTER_DISPATCH(filter_fwd_kernel, nb, tv, grid, dim3(256), lds, s, proj, filt_pad, taps, fp);
+  const size_t lds = filter_mfma_lds(fp.nu, false);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  dim3 grid((fp.nv + kFbm - 1) / kFbm, (fp.nu + kFbn - 1) / kFbn, fp.n_views);
+  hipLaunchKernelGGL(filter_fwd_mfma_kernel, grid, dim3(256), lds, s, proj, filt_pad, taps, fp);
   return hipGetLastError();
 }
 
 hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* taps, const FilterParams& fp,
                                hipStream_t s) {
-  const int nb = filter_nb(fp.nu);
-  if (nb < 0) return hipErrorInvalidValue;
-  const int tv = fp.nu <= 512 ? 16 : 8;
-  size_t lds = sizeof(float) * ((size_t)tv * (fp.nu + 1) + 2 * fp.nu - 1);
-  const size_t stage = sizeof(float) * (size_t)fp.nu * tv;
-  if (stage > lds) lds = stage;
-  dim3 grid((fp.nv + tv - 1) / tv, fp.n_views);
-  MONTE_FILTER_DISPATCH(filter_T_kernel, nb, tv, grid, dim3(256), lds, s, filt_pad, proj, taps, fp);
+  const size_t lds = filter_mfma_lds(fp.nu, true);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  dim3 grid((fp.nv + kFbm - 1) / kFbm, (fp.nu + kFbn - 1) / kFbn, fp.n_views);
+  hipLaunchKernelGGL(filter_T_mfma_kernel, grid, dim3(256), lds, s, filt_pad, proj, taps, fp);
   return hipGetLastError();
 }
 
