@@ -42,7 +42,8 @@ void* workspace(monte_ctx* c, int slot, size_t bytes) {
 namespace {
 
 constexpr int kSlotFwd = 0, kSlotAdj = 1, kSlotMc = 2, kSlotConst = 3, kSlotStageA = 4, kSlotStageB = 5,
-              kSlotTuneA = 6, kSlotTuneB = 7, kSlotVolT = 8;
+              kSlotTuneA = 6, kSlotTuneB = 7, kSlotVolT = 8, kSlotQuad = 9,
+              kSlotPlan = 10;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -99,6 +100,44 @@ int ensure_constants(monte_ctx* c, const monte_geom& g, const float** taps, cons
   }
   *taps = (const float*)((char*)c->ws[kSlotConst] + c->taps_off);
   *views = (const BpView*)((char*)c->ws[kSlotConst] + c->views_off);
+  return MONTE_OK;
+}
+
+// Build the projector's column lists once per geometry: a counting pass, the
+// prefix sums on the host, then a writing pass.
+int ensure_projector_plan(monte_ctx* c, const monte_geom& g, const BpView* views, const BpGrid& grid,
+                          const uint32_t** offs, const uint32_t** list) {
+  std::vector<double> key = geom_key(g);
+  key.push_back((double)project_strips(grid));
+  const size_t ncell = (size_t)g.n_views * project_strips(grid);
+  const size_t obytes = align_up((ncell + 1) * sizeof(uint32_t), 256);
+  if (key != c->plan_key || !c->ws[kSlotPlan]) {
+    c->plan_key.clear();
+    uint32_t* d = (uint32_t*)workspace(c, kSlotPlan, obytes);
+    if (!d) return fail(c, MONTE_ERR_NOMEM, "projector plan allocation failed");
+    MONTE_HIP(c, launch_project_plan(views, g.n_views, grid, d, nullptr, false, c->stream));
+    std::vector<uint32_t> h(ncell + 1);
+    MONTE_HIP(c, hipMemcpyAsync(h.data(), d, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    MONTE_HIP(c, hipStreamSynchronize(c->stream));
+    uint64_t total = 0;
+    for (size_t i = 0; i < ncell; ++i) {
+      const uint32_t n = h[i];
+      h[i] = (uint32_t)total;
+      total += n;
+      if (total > 0xffffffffull) return fail(c, MONTE_ERR_UNSUPPORTED, "projector plan exceeds 2^32 columns");
+    }
+    h[ncell] = (uint32_t)total;
+    char* base = (char*)workspace(c, kSlotPlan, obytes + total * sizeof(uint32_t));
+    if (!base) return fail(c, MONTE_ERR_NOMEM, "projector plan allocation failed");
+    MONTE_HIP(c, hipMemcpyAsync(base, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    MONTE_HIP(c, launch_project_plan(views, g.n_views, grid, (uint32_t*)base, (uint32_t*)(base + obytes), true,
+                                     c->stream));
+    MONTE_HIP(c, hipStreamSynchronize(c->stream));
+    c->plan_key = key;
+    c->plan_list_off = obytes;
+  }
+  *offs = (const uint32_t*)c->ws[kSlotPlan];
+  *list = (const uint32_t*)((char*)c->ws[kSlotPlan] + c->plan_list_off);
   return MONTE_OK;
 }
 
@@ -163,10 +202,13 @@ int run_backproject_padded(monte_ctx* c, const monte_geom& g, const float* pad, 
     MONTE_HIP(c, launch_fill(vol, 0.f, vol_elems(g), c->stream));
     accumulate = 1;  // ROI columns added onto zeros
   }
+  float4* quads = (float4*)workspace(c, kSlotQuad, padded_elems(g) * sizeof(float4));
+  if (!quads) return fail(c, MONTE_ERR_NOMEM, "tap quad allocation failed");
+  MONTE_HIP(c, launch_tap_quads(pad, quads, g.n_views, grid.rows, grid.pitch, c->stream));
   const int chunk = c->tuning.bp_view_chunk > 0 ? c->tuning.bp_view_chunk : g.n_views;
   for (int v0 = 0; v0 < g.n_views; v0 += chunk) {
     const int n = std::min(chunk, g.n_views - v0);
-    MONTE_HIP(c, launch_backproject(pad, views, v0, n, grid, vol, (accumulate || v0 > 0) ? 1 : 0, c->tuning,
+    MONTE_HIP(c, launch_backproject(quads, views, v0, n, grid, vol, (accumulate || v0 > 0) ? 1 : 0, c->tuning,
                                     c->stream));
   }
   return MONTE_OK;
@@ -178,6 +220,8 @@ int run_project_padded(monte_ctx* c, const monte_geom& g, const float* vol, floa
   int rc = ensure_constants(c, g, &taps, &views);
   if (rc) return rc;
   const BpGrid grid = compute_bp_grid(g);
+  const uint32_t *plan_offs, *plan_list;
+  if ((rc = ensure_projector_plan(c, g, views, grid, &plan_offs, &plan_list))) return rc;
   float* volT = (float*)workspace(c, kSlotVolT, vol_elems(g) * sizeof(float));
   if (!volT) return fail(c, MONTE_ERR_NOMEM, "transposed volume allocation failed");
   MONTE_HIP(c, launch_transpose_zx(vol, volT, g.nx, g.ny, g.nz, c->stream));
@@ -190,7 +234,8 @@ int run_project_padded(monte_ctx* c, const monte_geom& g, const float* vol, floa
     const int n = std::min(chunk, g.n_views - v0);
     int lds_r0, lds_rows;
     projector_rows(host_views, v0, n, grid, lds_r0, lds_rows);
-    MONTE_HIP(c, launch_project(volT, views, v0, n, grid, pad, lds_r0, lds_rows, c->tuning, c->stream));
+    MONTE_HIP(c, launch_project(volT, views, v0, n, grid, pad, lds_r0, lds_rows, plan_offs, plan_list, c->tuning,
+                                c->stream));
   }
   if (g.preset == MONTE_PRESET_BP3D20)
     MONTE_HIP(c, launch_fold_pads(pad, g.n_views, g.nu, g.nv, grid.pitch, grid.rows, c->stream));
@@ -395,7 +440,7 @@ monte_tuning default_tuning() {
   t.mc_block = 256;
   t.mc_batch = 16;
   t.mc_blocks_per_cu = 4;
-  t.bp_zchunk = 8;
+  t.bp_zchunk = 16;
   t.bp_tile_x = 8;   // fastest at 325^3 / 512^2 / 360 views
   t.bp_tile_y = 32;
   t.bp_view_chunk = 0;
@@ -894,10 +939,10 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
     const BpView* views;
     if ((rc = ensure_constants(c, tg, &taps, &views))) return rc;
     const int nviews = std::min(g->n_views, 360);  // L2 reuse across views shapes the best tile
-    float* pad = (float*)workspace(c, kSlotFwd, padded_elems(tg) * sizeof(float));
+    float4* quads = (float4*)workspace(c, kSlotQuad, padded_elems(tg) * sizeof(float4));
     float* vol = (float*)workspace(c, kSlotTuneB, vol_elems(tg) * sizeof(float));
-    if (!pad || !vol) return fail(c, MONTE_ERR_NOMEM, "tuning buffers");
-    MONTE_HIP(c, launch_fill(pad, 0.f, padded_elems(tg), c->stream));
+    if (!quads || !vol) return fail(c, MONTE_ERR_NOMEM, "tuning buffers");
+    MONTE_HIP(c, launch_fill((float*)quads, 0.f, 4 * padded_elems(tg), c->stream));
     const BpGrid grid = compute_bp_grid(tg);
     hipEvent_t e0, e1;
     MONTE_HIP(c, hipEventCreate(&e0));
@@ -911,9 +956,9 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
         t.bp_zchunk = zc;
         t.bp_tile_x = tx;
         t.bp_tile_y = 256 / tx;
-        MONTE_HIP(c, launch_backproject(pad, views, 0, nviews, grid, vol, 0, t, c->stream));  // warm
+        MONTE_HIP(c, launch_backproject(quads, views, 0, nviews, grid, vol, 0, t, c->stream));  // warm
         MONTE_HIP(c, hipEventRecord(e0, c->stream));
-        MONTE_HIP(c, launch_backproject(pad, views, 0, nviews, grid, vol, 0, t, c->stream));
+        MONTE_HIP(c, launch_backproject(quads, views, 0, nviews, grid, vol, 0, t, c->stream));
         MONTE_HIP(c, hipEventRecord(e1, c->stream));
         MONTE_HIP(c, hipEventSynchronize(e1));
         float ms = 0.f;
@@ -954,7 +999,9 @@ int monte_fdk_graph(monte_ctx* c, const monte_geom* g, const float* proj, float*
   if ((rc = ensure_constants(c, *g, &taps, &views))) return rc;
   float* pad = (float*)workspace(c, kSlotFwd, padded_elems(*g) * sizeof(float));
   if (!pad) return fail(c, MONTE_ERR_NOMEM, "filtered stack allocation failed");
+  if (!workspace(c, kSlotQuad, padded_elems(*g) * sizeof(float4))) return fail(c, MONTE_ERR_NOMEM, "tap quad allocation failed");
   key.push_back((double)(uintptr_t)pad);
+  key.push_back((double)(uintptr_t)c->ws[kSlotQuad]);
   key.push_back((double)(uintptr_t)c->ws[kSlotConst]);
   if (key != c->graph_key || !c->graph_exec) {
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }

@@ -288,23 +288,35 @@ hipError_t launch_fill_pads(float* buf, int n_views, int nu, int nv, int pitch, 
 }
 
 // Transpose of the bp3d20 guard-cell wrap: what the matched projector
-// deposited on a guard cell belongs to the interior cell it aliases.
+// deposited on a guard cell belongs to the interior cell it aliases.  An
+// interior cell (k, r, c) is aliased by at most three guard cells (for
+// nu >= kPadHi and nv > kPadHi):
+//   (k, r - 1, nu + c)        when c < kPadHi and r >= 1,
+//   (k - 1, nv + r - 1, nu + c) when c < kPadHi and r <= kPadHi,
+//   (k - 1, nv + r, c)        when r < kPadHi,
+// the inverse of wrap_source.  Each interior cell gathers them in that fixed
+// order, so the sum does not depend on scheduling.
 __global__ void fold_pads_kernel(float* __restrict__ buf, int n_views, int nu, int nv, int pitch, int rows) {
-  const size_t n = (size_t)n_views * rows * pitch;
+  const size_t n = (size_t)n_views * nv * nu;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int k = (int)(i / ((size_t)rows * pitch));
-    const int rem = (int)(i - (size_t)k * rows * pitch);
-    const int r = rem / pitch - kPadLo, c = rem % pitch - kPadLo;
-    if (r >= 0 && r < nv && c >= 0 && c < nu) continue;
-    int kk, rr, cc;
-    const float v = buf[i];
-    if (v != 0.f && wrap_source(k, r, c, n_views, nu, nv, kk, rr, cc))
-      atomicAdd(buf + ((size_t)kk * rows + rr + kPadLo) * pitch + cc + kPadLo, v);
+    const int k = (int)(i / ((size_t)nv * nu));
+    const int rem = (int)(i - (size_t)k * nv * nu);
+    const int r = rem / nu, c = rem - r * nu;
+    if (c >= kPadHi && r > kPadHi) continue;
+    auto at = [&](int kk, int rr, int cc) -> float& {
+      return buf[((size_t)kk * rows + rr + kPadLo) * pitch + cc + kPadLo];
+    };
+    float v = at(k, r, c);
+    if (c < kPadHi && r >= 1) v += at(k, r - 1, nu + c);
+    if (k >= 1 && c < kPadHi && r <= kPadHi) v += at(k - 1, nv + r - 1, nu + c);
+    if (k >= 1 && r < kPadHi) v += at(k - 1, nv + r, c);
+    at(k, r, c) = v;
   }
 }
 
 hipError_t launch_fold_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, hipStream_t s) {
-  const size_t n = (size_t)n_views * rows * pitch;
+  if (nu < kPadHi || nv <= kPadHi) return hipErrorInvalidValue;
+  const size_t n = (size_t)n_views * nv * nu;
   size_t blocks = (n + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(fold_pads_kernel, dim3((unsigned)blocks), dim3(256), 0, s, buf, n_views, nu, nv, pitch, rows);
@@ -394,15 +406,22 @@ __device__ inline ChunkBase chunk_base(const ColumnCore& cc, const BpView& c, do
   return b;
 }
 
-// Sample position of voxel zb + j: row index and fraction, and whether it
-// lies in the v window.  (Identical in both directions.)
-__device__ inline void chunk_row(const ColumnCore& cc, const ChunkBase& b, int j, int& row, float& fv, bool& okv) {
-  float f = fmaf((float)j, cc.dsv, b.fvb);
-  okv = f >= b.flo && f <= b.fhi;
-  f = fminf(fmaxf(f, b.flo), b.fhi);
-  const float fl = floorf(f);
-  fv = f - fl;
-  row = b.ivb + (int)fl;
+// Sample position of voxel zb + j, as the byte offset of its tap quad in the
+// view (`off0` is that of row ivb, column iu; rows are `prow` bytes apart) and
+// the row fraction; false when the sample lies outside the v window.  The
+// clamped coordinate equals the unclamped one exactly when it is inside, so
+// one compare gives the window test.  |floor(f)| < 2^23 (sv is clamped to
+// +-1e6 and the window is a few thousand rows) and prow < 2^23, so a 24-bit
+// multiply forms the row step; the sum is taken mod 2^32 and the true offset
+// lies inside the view.
+__device__ inline bool chunk_tap(const ColumnCore& cc, const ChunkBase& b, int j, uint32_t off0, uint32_t prow,
+                                 uint32_t& off, float& fv) {
+  const float f = fmaf((float)j, cc.dsv, b.fvb);
+  const float fc = fminf(fmaxf(f, b.flo), b.fhi);
+  const float fl = floorf(fc);
+  fv = fc - fl;
+  off = off0 + (uint32_t)__mul24((int)fl, (int)prow);
+  return fc == f;
 }
 
 __device__ inline bool in_mask(const BpGrid& g, int x, int y, int z) {
@@ -416,7 +435,7 @@ __device__ inline double voxel_x(const BpGrid& g, int x) { return __dadd_rn(g.gx
 __device__ inline double voxel_y(const BpGrid& g, int y) { return __dadd_rn(g.gy0, __dmul_rn((double)y, g.gsy)); }
 
 template <int ZC>
-__global__ __launch_bounds__(256) void backproject_kernel(const float* __restrict__ filt,
+__global__ __launch_bounds__(256) void backproject_kernel(const float4* __restrict__ quads,
                                                           const BpView* __restrict__ views, int view0, int nviews,
                                                           BpGrid g, float* __restrict__ vol, int accumulate,
                                                           int tile_x, int tiles_x, int tiles_y) {
@@ -440,23 +459,26 @@ __global__ __launch_bounds__(256) void backproject_kernel(const float* __restric
 #pragma unroll
   for (int j = 0; j < ZC; ++j) acc[j] = 0.f;
 
+  // One 16-byte load per voxel: the vector memory pipeline costs about the
+  // same per instruction whatever its width, so the four bilinear taps come
+  // from one tap quad rather than two 8-byte row pairs.
   const size_t view_stride = (size_t)g.rows * g.pitch;
+  const uint32_t prow = (uint32_t)g.pitch * 16u;
   for (int k = view0; k < view0 + nviews; ++k) {
     const BpView& c = views[k];
     const ColumnCore cc = column_core(c, X, Y, Xf, Yf, g);
     const ChunkBase cb = chunk_base(cc, c, Zb, g);
-    const float* base = filt + (size_t)k * view_stride + (cc.iu + kPadLo);
+    const char* vb = (const char*)(quads + (size_t)k * view_stride);
+    const uint32_t off0 = (uint32_t)(cb.ivb + kPadLo) * prow + (uint32_t)(cc.iu + kPadLo) * 16u;
     const float wu = cc.ok ? cc.w : 0.f;
 #pragma unroll
     for (int j = 0; j < ZC; ++j) {
-      int row;
+      uint32_t off;
       float fv;
-      bool okv;
-      chunk_row(cc, cb, joff + j, row, fv, okv);
-      const float* p = base + (size_t)(row + kPadLo) * g.pitch;
-      const float p00 = p[0], p01 = p[1], p10 = p[g.pitch], p11 = p[g.pitch + 1];
-      const float a = fmaf(cc.fu, p01 - p00, p00);
-      const float b = fmaf(cc.fu, p11 - p10, p10);
+      const bool okv = chunk_tap(cc, cb, joff + j, off0, prow, off, fv);
+      const float4 t = *(const float4*)(vb + off);
+      const float a = fmaf(cc.fu, t.y - t.x, t.x);
+      const float b = fmaf(cc.fu, t.w - t.z, t.z);
       const float val = fmaf(fv, b - a, a);
       acc[j] = fmaf((okv && live[j]) ? wu : 0.f, val, acc[j]);
     }
@@ -480,21 +502,53 @@ static void bp_tiling(const BpGrid& grid, const monte_tuning& t, int zc, int& ti
   nblocks = (long)tiles_x * tiles_y * zchunks;
 }
 
-hipError_t launch_backproject(const float* filt_pad, const BpView* views, int view0, int nviews_chunk,
+// quads [view][rows][pitch]: quad (r, c) = (p[r][c], p[r][c + 1], p[r + 1][c], p[r + 1][c + 1])
+// of the padded filtered view p, zero past the last row or column.
+__global__ void tap_quads_kernel(const float* __restrict__ pad, float4* __restrict__ quads, int rows, int pitch,
+                                 size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t per_view = (size_t)rows * pitch;
+    const size_t k = i / per_view;
+    const int rc = (int)(i - k * per_view);
+    const int r = rc / pitch, cl = rc - r * pitch;
+    const float* p = pad + i;
+    const bool rr = r + 1 < rows, cr = cl + 1 < pitch;
+    float4 q;
+    q.x = p[0];
+    q.y = cr ? p[1] : 0.f;
+    q.z = rr ? p[pitch] : 0.f;
+    q.w = (rr && cr) ? p[pitch + 1] : 0.f;
+    quads[i] = q;
+  }
+}
+
+hipError_t launch_tap_quads(const float* filt_pad, float4* quads, int n_views, int rows, int pitch, hipStream_t s) {
+  const size_t n = (size_t)n_views * rows * pitch;
+  if (!n) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(tap_quads_kernel, dim3((unsigned)blocks), dim3(256), 0, s, filt_pad, quads, rows, pitch, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_backproject(const float4* quads, const BpView* views, int view0, int nviews_chunk,
                               const BpGrid& grid, float* vol, int accumulate, const monte_tuning& t,
                               hipStream_t s) {
   const int zc = t.bp_zchunk > 0 ? t.bp_zchunk : 8;
   int tile_x, tiles_x, tiles_y;
   long nblocks;
   if (t.bp_tile_x > 0 && 256 % t.bp_tile_x) return hipErrorInvalidValue;
+  // quads are addressed by 32-bit byte offsets with a 24-bit row multiply
+  if ((long long)grid.pitch * 16 >= (1LL << 23) || (long long)grid.rows * grid.pitch * 16 >= (1LL << 32))
+    return hipErrorInvalidValue;
   bp_tiling(grid, t, zc, tile_x, tiles_x, tiles_y, nblocks);
   if (nblocks <= 0) return hipSuccess;
   if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
   dim3 gd((unsigned)nblocks), bd(256);
   switch (zc) {
-    case 4: hipLaunchKernelGGL((backproject_kernel<4>), gd, bd, 0, s, filt_pad, views, view0, nviews_chunk, grid, vol, accumulate, tile_x, tiles_x, tiles_y); break;
-    case 8: hipLaunchKernelGGL((backproject_kernel<8>), gd, bd, 0, s, filt_pad, views, view0, nviews_chunk, grid, vol, accumulate, tile_x, tiles_x, tiles_y); break;
-    case 16: hipLaunchKernelGGL((backproject_kernel<16>), gd, bd, 0, s, filt_pad, views, view0, nviews_chunk, grid, vol, accumulate, tile_x, tiles_x, tiles_y); break;
+    case 4: hipLaunchKernelGGL((backproject_kernel<4>), gd, bd, 0, s, quads, views, view0, nviews_chunk, grid, vol, accumulate, tile_x, tiles_x, tiles_y); break;
+    case 8: hipLaunchKernelGGL((backproject_kernel<8>), gd, bd, 0, s, quads, views, view0, nviews_chunk, grid, vol, accumulate, tile_x, tiles_x, tiles_y); break;
+    case 16: hipLaunchKernelGGL((backproject_kernel<16>), gd, bd, 0, s, quads, views, view0, nviews_chunk, grid, vol, accumulate, tile_x, tiles_x, tiles_y); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -503,9 +557,10 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 // ------------------------------------------- transposed backprojection
 // The matched projector, written as a gather so no atomics are needed.
 // One wave owns a strip of kStripW padded detector columns (all rows) of one
-// view, held in its own LDS.  It enumerates the voxel columns whose left tap
+// view, held in its own LDS.  It walks the voxel columns whose left tap
 // lands in the strip (each voxel column of the view belongs to exactly one
-// strip; its right tap may spill into the next strip's first column): for each slab of the volume across the view's rays
+// strip; its right tap may spill into the next strip's first column), from a
+// list enumerated once per geometry: for each slab of the volume across the view's rays
 // (x-slabs when |cos b| >= |sin b|, y-slabs otherwise) the projective map
 // su(X,Y) is inverted at the strip's two edges, giving a range of columns on
 // that slab (widened by one voxel each side).  The lanes evaluate the
@@ -527,12 +582,13 @@ hipError_t launch_backproject(const float* filt_pad, const BpView* views, int vi
 // wave's LDS operations execute in program order, so updates of a cell by
 // neighbouring lanes, later voxels and later columns all see each other.
 // LDS column 0 is unused and column kStripW + 1 collects the right-tap
-// spill into the next strip.  The row pitch sets how the lanes' strided rows fall on
-// LDS banks: at 325^3 / 512^2 a pitch of 7 ran 20% faster than 9 and 30%
-// faster than the even pitches 6 and 10, and a 5-column strip balances the
-// per-strip enumeration against the waves a CU can hold.
-constexpr int kStripW = 5;
-constexpr int kStripP = kStripW + 2;
+// spill into the next strip.  The kernel is bound by LDS latency, so the
+// number of waves a CU holds sets its speed, and a wave's LDS is its strip:
+// with the column lists built once per geometry (below), one-column strips
+// (pitch 3) ran 13.1 ms at 325^3 / 512^2 / 360 views, against 14.7, 15.2 and
+// 16.6 ms for strips of 2, 3 and 5 columns.
+constexpr int kStripW = 1;
+constexpr int kStripP = 3;
 
 // Columns of one strip on each slab.  The strip's two edges su = s_lo and
 // su = s_hi are lines through the source; on a slab (x fixed for x-slabs, y
@@ -801,33 +857,71 @@ struct Enumerator {
   }
 };
 
+// The strip's geometry in a view: owned detector columns and slab setup.
+struct StripSetup {
+  int pu0, u_lo, u_hi, n_slabs;
+  bool x_slabs;
+  double s_lo, s_hi;
+  __device__ StripSetup(const BpView& c, const BpGrid& g, int st) {
+    pu0 = st * kStripW;   // first padded column of the strip
+    u_lo = pu0 - kPadLo;  // first detector column index owned
+    u_hi = u_lo + kStripW - 1;
+    x_slabs = fabs(c.cx) >= fabs(c.cy);
+    // columns whose left tap iu satisfies u_lo <= iu <= u_hi, i.e. su in [u_lo, u_hi + 1)
+    s_lo = (double)u_lo - 0.5;
+    s_hi = (double)(u_hi + 1) + 0.5;
+    n_slabs = x_slabs ? (g.x1 - g.x0) : g.ny;
+  }
+};
+
+// Column lists of the matched projector, built once per geometry: for view k
+// and strip st, the voxel columns whose left tap lands in the strip, in the
+// enumeration order above, packed as x | y << 16.  The first pass (WRITE
+// false) stores each list's length at offs[k * n_strips + st]; the second
+// writes the list at list + offs[k * n_strips + st] (offsets by then).
+template <bool WRITE>
+__global__ __launch_bounds__(64) void project_plan_kernel(const BpView* __restrict__ views, int n_strips, BpGrid g,
+                                                          uint32_t* __restrict__ offs, uint32_t* __restrict__ list) {
+  const int k = blockIdx.x / n_strips;
+  const int st = blockIdx.x - k * n_strips;
+  const int lane = threadIdx.x;
+  const BpView& c = views[k];
+  const StripSetup ss(c, g, st);
+  Enumerator en;
+  en.sr.init(ss.s_lo, ss.s_hi, c, ss.x_slabs, g);
+  ColumnBatch cb;
+  const uint32_t base = WRITE ? offs[blockIdx.x] : 0u;
+  uint32_t n = 0;
+  while (en.fill(cb, c, g, ss.x_slabs, ss.n_slabs, ss.u_lo, ss.u_hi, lane)) {
+    if (WRITE && lane < cb.n) list[base + n + lane] = (uint32_t)cb.x | ((uint32_t)cb.y << 16);
+    n += (uint32_t)cb.n;
+  }
+  if (!WRITE && lane == 0) offs[blockIdx.x] = n;
+}
+
 template <int C, bool MASK>
 __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ volT,
                                                      const BpView* __restrict__ views, int view0, int n_strips,
-                                                     BpGrid g, float* __restrict__ filt, int lds_r0, int lds_rows) {
+                                                     BpGrid g, float* __restrict__ filt, int lds_r0, int lds_rows,
+                                                     const uint32_t* __restrict__ offs,
+                                                     const uint32_t* __restrict__ list) {
   // [lds_rows][kStripP] (padded rows lds_r0.., column q = padded column pu0 + q - 1), then kTrash floats
   extern __shared__ float strip[];
   const int k = view0 + blockIdx.x / n_strips;
   const int st = blockIdx.x - (k - view0) * n_strips;
-  const int pu0 = st * kStripW;      // first padded column of the strip
-  const int u_lo = pu0 - kPadLo;     // first detector column index owned
-  const int u_hi = u_lo + kStripW - 1;
   const int lane = threadIdx.x;
   const int trash = lds_rows * kStripP;
   for (int i = lane; i < trash + kTrash; i += 64) strip[i] = 0.f;
   __syncthreads();
 
   const BpView& c = views[k];
-  const bool x_slabs = fabs(c.cx) >= fabs(c.cy);
-  // columns whose left tap iu satisfies u_lo <= iu <= u_hi, i.e. su in [u_lo, u_hi + 1)
-  const double s_lo = (double)u_lo - 0.5, s_hi = (double)(u_hi + 1) + 0.5;
-  const int n_slabs = x_slabs ? (g.x1 - g.x0) : g.ny;
+  const int pu0 = st * kStripW;  // first padded column of the strip
   const int nch = (g.nz + C - 1) / C;
   const int npass = (nch + 63) / 64;
   constexpr int NP = ColumnVals<C>::NP;
+  const size_t cell = (size_t)k * n_strips + st;
+  const uint32_t beg = offs[cell], end = offs[cell + 1];
 
-  Enumerator en;
-  en.sr.init(s_lo, s_hi, c, x_slabs, g);
   ColumnBatch cb;
   // The volume values of the next kAhead columns are in flight while one is
   // accumulated: a column's accumulation takes less time than a load from HBM.
@@ -861,7 +955,22 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
       }
     }
   };
-  while (en.fill(cb, c, g, x_slabs, n_slabs, u_lo, u_hi, lane)) {
+  // Batches of 64 listed columns, lane i holding the i-th with the same column
+  // arithmetic the enumeration used to select it.
+  for (uint32_t b0 = beg; b0 < end; b0 += 64) {
+    cb.n = (int)min(64u, end - b0);
+    if (lane < cb.n) {
+      const uint32_t e = list[b0 + lane];
+      cb.x = (int)(e & 0xffffu);
+      cb.y = (int)(e >> 16);
+      const double X = voxel_x(g, cb.x), Y = voxel_y(g, cb.y);
+      const ColumnCore mine = column_core(c, X, Y, (float)X, (float)Y, g);
+      cb.iu = mine.iu;
+      cb.fu = mine.fu;
+      cb.w = mine.w;
+      cb.dsv = mine.dsv;
+      cb.kv = mine.kv;
+    }
     ColumnVals<C> ring[kAhead];
 #pragma unroll
     for (int q = 0; q < kAhead; ++q)
@@ -926,9 +1035,26 @@ hipError_t launch_transpose_zx(const float* vol, float* volT, int nx, int ny, in
   return hipGetLastError();
 }
 
+int project_strips(const BpGrid& grid) { return (grid.pitch + kStripW - 1) / kStripW; }
+
+hipError_t launch_project_plan(const BpView* views, int n_views, const BpGrid& grid, uint32_t* offs, uint32_t* list,
+                               bool write, hipStream_t s) {
+  const long nblocks = (long)project_strips(grid) * n_views;
+  if (nblocks <= 0) return hipSuccess;
+  if (nblocks > 0x7fffffffL || grid.nx > 65536 || grid.ny > 65536) return hipErrorInvalidValue;
+  if (write)
+    hipLaunchKernelGGL(project_plan_kernel<true>, dim3((unsigned)nblocks), dim3(64), 0, s, views,
+                       project_strips(grid), grid, offs, list);
+  else
+    hipLaunchKernelGGL(project_plan_kernel<false>, dim3((unsigned)nblocks), dim3(64), 0, s, views,
+                       project_strips(grid), grid, offs, list);
+  return hipGetLastError();
+}
+
 hipError_t launch_project(const float* volT, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
-                          float* filt_pad, int lds_r0, int lds_rows, const monte_tuning& t, hipStream_t s) {
-  const int n_strips = (grid.pitch + kStripW - 1) / kStripW;
+                          float* filt_pad, int lds_r0, int lds_rows, const uint32_t* plan_offs,
+                          const uint32_t* plan_list, const monte_tuning& t, hipStream_t s) {
+  const int n_strips = project_strips(grid);
   const long nblocks = (long)n_strips * nviews_chunk;
   if (nblocks <= 0) return hipSuccess;
   if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
@@ -947,7 +1073,7 @@ hipError_t launch_project(const float* volT, const BpView* views, int view0, int
   const dim3 gd((unsigned)nblocks), bd(64);
 #define MONTE_PJ(CZ, MK)                                                                                      \
   hipLaunchKernelGGL((project_kernel<CZ, MK>), gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad, \
-                     lds_r0, lds_rows)
+                     lds_r0, lds_rows, plan_offs, plan_list)
   const bool mk = grid.mask != 0;
   switch (best) {
     case 2: if (mk) MONTE_PJ(2, true); else MONTE_PJ(2, false); break;

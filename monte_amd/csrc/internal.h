@@ -146,13 +146,19 @@ struct monte_ctx {
   //   2 MC descriptors                    3 FDK constants (taps + per-view coefficients)
   //   4..7 staging for host-pointer mode and tuning scratch
   //   8 z-fastest copy of the volume for the matched projector
-  static constexpr int kSlots = 9;
+  //   9 tap quads of the forward filtered stack (backprojection input)
+  //  10 column lists of the matched projector (per geometry)
+  static constexpr int kSlots = 11;
   void* ws[kSlots] = {};
   size_t ws_size[kSlots] = {};
 
   // FDK constants cached by geometry
   std::vector<double> const_key;
   size_t taps_off = 0, views_off = 0;
+
+  // projector column lists cached by geometry: offsets, then the lists
+  std::vector<double> plan_key;
+  size_t plan_list_off = 0;
 
   // hipGraph cache for monte_fdk_graph
   hipGraphExec_t graph_exec = nullptr;
@@ -185,14 +191,26 @@ hipError_t launch_unpad_copy(const float* filt_pad, float* filt, int n_views, in
                              hipStream_t s);
 hipError_t launch_fill_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, int wrap, hipStream_t s);
 hipError_t launch_fold_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, hipStream_t s);
-hipError_t launch_backproject(const float* filt_pad, const BpView* views, int view0, int nviews_chunk,
+// Tap quads of the padded filtered views: quads[i] holds the 2 x 2 bilinear
+// neighbourhood whose top-left cell is padded cell i (n_views * rows * pitch).
+hipError_t launch_tap_quads(const float* filt_pad, float4* quads, int n_views, int rows, int pitch, hipStream_t s);
+hipError_t launch_backproject(const float4* quads, const BpView* views, int view0, int nviews_chunk,
                               const BpGrid& grid, float* vol, int accumulate, const monte_tuning& t, hipStream_t s);
 // volT: the volume with z fastest ([y][x][z], see launch_transpose_zx); every
 // cell of the padded views [view0, view0 + nviews_chunk) is written.  Padded
 // rows [lds_r0, lds_r0 + lds_rows) must hold every row the views' voxels can
 // reach (projector_rows); the others are written as zeros.
+// Detector strips per view of the matched projector.
+int project_strips(const BpGrid& grid);
+// Column lists of the projector for views [0, n_views): with write false,
+// offs[k * project_strips + st] receives each list's length; with write true,
+// offs holds the exclusive prefix sums (plus the total at the end) and the
+// lists are written to list.
+hipError_t launch_project_plan(const BpView* views, int n_views, const BpGrid& grid, uint32_t* offs, uint32_t* list,
+                               bool write, hipStream_t s);
 hipError_t launch_project(const float* volT, const BpView* views, int view0, int nviews_chunk, const BpGrid& grid,
-                          float* filt_pad, int lds_r0, int lds_rows, const monte_tuning& t, hipStream_t s);
+                          float* filt_pad, int lds_r0, int lds_rows, const uint32_t* plan_offs,
+                          const uint32_t* plan_list, const monte_tuning& t, hipStream_t s);
 // Padded detector rows reachable by the voxels of views [view0, view0 + n).
 void projector_rows(const std::vector<BpView>& views, int view0, int n, const BpGrid& grid, int& lds_r0,
                     int& lds_rows);
