@@ -581,14 +581,38 @@ hipError_t launch_backproject(const float4* quads, const BpView* views, int view
 // row is updated by a second read-add-write after the first completes, and a
 // wave's LDS operations execute in program order, so updates of a cell by
 // neighbouring lanes, later voxels and later columns all see each other.
-// LDS column 0 is unused and column kStripW + 1 collects the right-tap
-// spill into the next strip.  The kernel is bound by LDS latency, so the
-// number of waves a CU holds sets its speed, and a wave's LDS is its strip:
-// with the column lists built once per geometry (below), one-column strips
-// (pitch 3) ran 13.1 ms at 325^3 / 512^2 / 360 views, against 14.7, 15.2 and
-// 16.6 ms for strips of 2, 3 and 5 columns.
+// LDS column kStripC0 + kStripW collects the right-tap spill into the next
+// strip.  The kernel's speed follows the number of waves a CU holds, and a
+// wave's LDS is its strip: with the column lists built once per geometry
+// (below), one-column strips ran 13.1 ms at 325^3 / 512^2 / 360 views,
+// against 14.7, 15.2 and 16.6 ms for strips of 2, 3 and 5 columns.  A row is
+// then the pair (left tap, spill), read and written as one 8-byte access
+// (11.8 ms, against 12.3 ms for pitch 3 and 13.0 ms for pitch 4).
 constexpr int kStripW = 1;
-constexpr int kStripP = 3;
+constexpr int kStripP = 2;
+constexpr int kStripC0 = 0;  // LDS column of the strip's first column
+
+// A cell and its right neighbour (left and right tap of one row), as one
+// 8-byte LDS access when the layout keeps them 8-byte aligned.
+constexpr bool kPairAligned = kStripP % 2 == 0 && kStripC0 % 2 == 0;
+__device__ inline float2 lds_get2(const float* strip, int i) {
+  if constexpr (kPairAligned) return *(const float2*)(strip + i);
+  return make_float2(strip[i], strip[i + 1]);
+}
+__device__ inline void lds_put2(float* strip, int i, float2 v) {
+  if constexpr (kPairAligned) {
+    *(float2*)(strip + i) = v;
+  } else {
+    strip[i] = v.x;
+    strip[i + 1] = v.y;
+  }
+}
+__device__ inline void lds_add2(float* strip, int i, float a, float b) {
+  float2 x = lds_get2(strip, i);
+  x.x += a;
+  x.y += b;
+  lds_put2(strip, i, x);
+}
 
 // Columns of one strip on each slab.  The strip's two edges su = s_lo and
 // su = s_hi are lines through the source; on a slab (x fixed for x-slabs, y
@@ -643,7 +667,7 @@ __device__ inline void lds_order() { asm volatile("" ::: "memory"); }
 // Passes (64 chunks of C voxels) whose volume values are fetched one column
 // ahead; columns longer than kPrefetchZ / C / 64 passes load the rest in place.
 constexpr int kPrefetchZ = 384;
-constexpr int kAhead = 4;
+constexpr int kAhead = 2;
 // LDS floats after the strip that absorb the updates of lanes with nothing to
 // add (outside the v window or past the column end), so the updates need no
 // branches: two per lane, two row pitches and one cell of overlap.  (Branches
@@ -722,11 +746,9 @@ __device__ inline void pass_updates(float* __restrict__ strip, const float (&v)[
     const float bb = sw * fv, aa = sw - bb;
     const float a1 = aa * cc.fu, b1 = bb * cc.fu;
     const int i0 = live ? cell0 + row * kStripP : trash;
-    strip[i0] += aa - a1;
-    strip[i0 + 1] += a1;
+    lds_add2(strip, i0, aa - a1, a1);
     lds_order();
-    strip[i0 + kStripP] += bb - b1;
-    strip[i0 + kStripP + 1] += b1;
+    lds_add2(strip, i0 + kStripP, bb - b1, b1);
     lds_order();
   }
 }
@@ -736,9 +758,13 @@ __device__ inline void pass_updates(float* __restrict__ strip, const float (&v)[
 // fold into three consecutive rows r, r + 1, r + 2, added in three rounds.  In
 // round k every lane updates its row r + k; the lanes' rows r strictly
 // increase, so no two lanes of a round touch the same cell.
-__device__ inline void pass_updates_pair(float* __restrict__ strip, const float (&v)[2], const ColumnCore& cc,
-                                         long long s0, long long d, long long lo, long long hi, int z0, int cell0,
-                                         int trash, const BpGrid& g) {
+struct PairRows {
+  int i0;            // LDS index of (row r, left tap), or the trash floats
+  float q0, q1, q2;  // totals for rows r, r + 1, r + 2
+};
+
+__device__ inline PairRows pair_rows(const float (&v)[2], const ColumnCore& cc, long long s0, long long d,
+                                     long long lo, long long hi, int z0, int cell0, int trash, const BpGrid& g) {
   const long long s1 = s0 + d;
   const int r0 = fx_row(s0), r1 = fx_row(s1);
   const bool live0 = s0 >= lo && s0 <= hi && z0 < g.nz, live1 = s1 >= lo && s1 <= hi && z0 + 1 < g.nz;
@@ -746,20 +772,50 @@ __device__ inline void pass_updates_pair(float* __restrict__ strip, const float 
   const float b0 = sw0 * fx_frac(s0), a0 = sw0 - b0;
   const float b1 = sw1 * fx_frac(s1), a1 = sw1 - b1;
   const bool same = r1 == r0;
-  const float q0 = a0 + (same ? a1 : 0.f);
-  const float q1 = b0 + (same ? b1 : a1);
-  const float q2 = same ? 0.f : b1;
-  const int i0 = (live0 || live1) ? cell0 + r0 * kStripP : trash;
-  const float t0 = q0 * cc.fu, t1 = q1 * cc.fu, t2 = q2 * cc.fu;
-  strip[i0] += q0 - t0;
-  strip[i0 + 1] += t0;
-  lds_order();
-  strip[i0 + kStripP] += q1 - t1;
-  strip[i0 + kStripP + 1] += t1;
-  lds_order();
-  strip[i0 + 2 * kStripP] += q2 - t2;
-  strip[i0 + 2 * kStripP + 1] += t2;
-  lds_order();
+  PairRows p;
+  p.q0 = a0 + (same ? a1 : 0.f);
+  p.q1 = b0 + (same ? b1 : a1);
+  p.q2 = same ? 0.f : b1;
+  p.i0 = (live0 || live1) ? cell0 + r0 * kStripP : trash;
+  return p;
+}
+
+__device__ inline float pair_q(const PairRows& p, int k) { return k == 0 ? p.q0 : (k == 1 ? p.q1 : p.q2); }
+
+__device__ inline void pass_updates_pair(float* __restrict__ strip, const float (&v)[2], const ColumnCore& cc,
+                                         long long s0, long long d, long long lo, long long hi, int z0, int cell0,
+                                         int trash, const BpGrid& g) {
+  const PairRows p = pair_rows(v, cc, s0, d, lo, hi, z0, cell0, trash, g);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int i = p.i0 + k * kStripP;
+    const float q = pair_q(p, k), t = q * cc.fu;
+    lds_add2(strip, i, q - t, t);
+    lds_order();
+  }
+}
+
+// Two pair passes whose rows cannot meet, run round by round together so each
+// round's two reads are in flight at once.  Passes p and p + 2 of a column
+// start 256 voxels apart, so at a v-slope above 0.505 rows per voxel the first
+// row of the later pass lies more than 64 rows above the last row (r + 2) of
+// the earlier.
+__device__ inline void pass_updates_pair2(float* __restrict__ strip, const float (&va)[2], const float (&vb)[2],
+                                          const ColumnCore& cc, long long sa, long long sb, long long d,
+                                          long long lo, long long hi, int za, int zb, int cell0, int trash,
+                                          const BpGrid& g) {
+  const PairRows a = pair_rows(va, cc, sa, d, lo, hi, za, cell0, trash, g);
+  const PairRows b = pair_rows(vb, cc, sb, d, lo, hi, zb, cell0, trash, g);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int ia = a.i0 + k * kStripP, ib = b.i0 + k * kStripP;
+    const float qa = pair_q(a, k), ta = qa * cc.fu;
+    const float qb = pair_q(b, k), tb = qb * cc.fu;
+    const float2 xa = lds_get2(strip, ia), xb = lds_get2(strip, ib);
+    lds_put2(strip, ia, make_float2(xa.x + (qa - ta), xa.y + ta));
+    lds_put2(strip, ib, make_float2(xb.x + (qb - tb), xb.y + tb));
+    lds_order();
+  }
 }
 
 // One pass of a column: lane l takes the C voxels of chunk ch0 + l.  `cell0`
@@ -905,7 +961,7 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
                                                      BpGrid g, float* __restrict__ filt, int lds_r0, int lds_rows,
                                                      const uint32_t* __restrict__ offs,
                                                      const uint32_t* __restrict__ list) {
-  // [lds_rows][kStripP] (padded rows lds_r0.., column q = padded column pu0 + q - 1), then kTrash floats
+  // [lds_rows][kStripP] (padded rows lds_r0.., column q = padded column pu0 + q - kStripC0), then kTrash floats
   extern __shared__ float strip[];
   const int k = view0 + blockIdx.x / n_strips;
   const int st = blockIdx.x - (k - view0) * n_strips;
@@ -940,8 +996,18 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
     const float step = (float)C * fabsf(cc.dsv);
     const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
     const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: 0 .. kStripW - 1
-    const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + 1;
+    const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + kStripC0;
     const ColumnFx fx = column_fx(cc, c, g);
+    if constexpr (C == 2) {
+      // three pair passes: the first and last together, then the middle one
+      if (npass == 3 && phases == 1 && cc.dsv > 0.f && cc.dsv < 0.999f) {
+        const int za = 2 * lane, zb = 2 * (128 + lane);
+        pass_updates_pair2(strip, vals.v[0], vals.v[2], cc, fx.a + (long long)za * fx.d,
+                           fx.a + (long long)zb * fx.d, fx.d, fx.lo, fx.hi, za, zb, cell0, trash + 2 * lane, g);
+        project_pass<C>(strip, vals.v[1], cc, fx, cell0, trash + 2 * lane, 64, phases, g, lane);
+        return;
+      }
+    }
 #pragma unroll
     for (int p = 0; p < NP; ++p)
       if (p < npass) project_pass<C>(strip, vals.v[p], cc, fx, cell0, trash + 2 * lane, p * 64, phases, g, lane);
@@ -1003,11 +1069,11 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
     const bool held = lr >= 0 && lr < lds_rows;
     if (t == 0 || t == kStripW) {
       if (held) {
-        const float val = strip[lr * kStripP + t + 1];
+        const float val = strip[lr * kStripP + t + kStripC0];
         if (val != 0.f) unsafeAtomicAdd(o, val);
       }
     } else {
-      *o = held ? strip[lr * kStripP + t + 1] : 0.f;
+      *o = held ? strip[lr * kStripP + t + kStripC0] : 0.f;
     }
   }
 }
