@@ -7,7 +7,7 @@
 
 `copy` copies M MiB device-to-device once (a known byte count, used to scale
 the FETCH_SIZE / WRITE_SIZE counters).  The others run one pass of a hot path
-on the full CBCT workload (325^3 phantom at 0.5 mm, 512^2 detector at 1.2 mm,
+(`adjoint`: matched projector + transposed filter) on the full CBCT workload (325^3 phantom at 0.5 mm, 512^2 detector at 1.2 mm,
 SOD 600 / SDD 1200 mm, 120 kVp spectrum) after one untimed warm-up pass, and
 print what ran, so the kernel of interest appears in the trace twice with the
 same arguments.
@@ -59,7 +59,7 @@ def main(argv=None):
                   f"wall {(time.perf_counter() - t) * 1e3:.1f} ms", flush=True)
         return 0
 
-    from .fdk import fdk, fdk_project
+    from .fdk import fdk, fdk_adjoint
 
     g = torch.Generator(device=dev).manual_seed(0)
     if a.what == "fdk":
@@ -67,7 +67,7 @@ def main(argv=None):
         fn = fdk
     else:
         x = torch.randn(geom.vol_shape, device=dev, generator=g)
-        fn = fdk_project
+        fn = fdk_adjoint
     for rep in range(2):
         t = time.perf_counter()
         fn(x, geom, ctx)
