@@ -270,6 +270,8 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
     return fail(c, MONTE_ERR_INVALID, "bad phantom dimensions");
   if ((double)ph->nx * ph->ny * ph->nz > 4294967295.0)
     return fail(c, MONTE_ERR_INVALID, "phantom has more than 2^32 - 1 voxels");
+  if ((double)ph->ny * ph->nz > 16777216.0 || ph->nx > 16777216)
+    return fail(c, MONTE_ERR_UNSUPPORTED, "phantom ny * nz and nx must not exceed 2^24 (24-bit voxel indexing)");
   if (ph->n_labels < 1 || ph->n_labels > kMaxLabels) return fail(c, MONTE_ERR_INVALID, "n_labels out of range");
   if (c->n_tables < 1) return fail(c, MONTE_ERR_STATE, "attenuation tables not loaded (monte_load_tables)");
   if (cfg->photons_per_view == 0 || cfg->photons_per_view > 0xFFFFFFFFull)

@@ -261,16 +261,30 @@ __device__ inline float mu_at(const Photon& p, int lab, const McJob& J) {
   return mu;
 }
 
-__device__ inline int label_at(const Photon& p, float t, const McJob& J, float& qx, float& qy, float& qz) {
+// a * b + c with 24-bit a, b, at full rate (the compiler otherwise picks the
+// quarter-rate 64-bit multiply-add when it cannot prove the operand widths)
+__device__ inline uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
+
+// Position at flight parameter t and the index of its voxel.
+__device__ inline uint32_t voxel_at(const Photon& p, float t, const McJob& J, float& qx, float& qy, float& qz) {
   qx = fmaf(t, p.dx, p.px); qy = fmaf(t, p.dy, p.py); qz = fmaf(t, p.dz, p.pz);
   // voxel index = (q - b0) / v as one FMA; truncation equals floor wherever
   // the clamp to [0, n - 1] does not already decide it
   const int ix = min(max((int)fmaf(qx, J.inv_vx, J.ix0), 0), J.nx - 1);
   const int iy = min(max((int)fmaf(qy, J.inv_vy, J.iy0), 0), J.ny - 1);
   const int iz = min(max((int)fmaf(qz, J.inv_vz, J.iz0), 0), J.nz - 1);
-  // 32-bit voxel index (checked on the host) and a global-address-space load;
-  // labels past the table are clamped so no table read leaves it
-  const uint32_t i = ((uint32_t)iz * (uint32_t)J.ny + (uint32_t)iy) * (uint32_t)J.nx + (uint32_t)ix;
+  // 32-bit index from two 24-bit multiply-adds: the host checks
+  // nz * ny <= 2^24, nx <= 2^24 and nx * ny * nz < 2^32
+  return mad_u24(mad_u24((uint32_t)iz, (uint32_t)J.ny, (uint32_t)iy), (uint32_t)J.nx, (uint32_t)ix);
+}
+
+// Label of a voxel, by a global-address-space load; labels past the table are
+// clamped so no table read leaves it.
+__device__ inline int label_of(uint32_t i, const McJob& J) {
   return min((int)((const __attribute__((address_space(1))) uint8_t*)J.labels)[i], kMaxLabels - 1);
 }
 
@@ -489,8 +503,9 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       const float ta = fmaf(-__builtin_amdgcn_logf(u01(c0)), p.lscale, p.t);
       const float tb = fmaf(-__builtin_amdgcn_logf(u01(c2)), p.lscale, ta);
       float ax, ay, az, bx, by, bz;
-      const int la = label_at(p, ta, *Jp, ax, ay, az);
-      const int lb = label_at(p, tb, *Jp, bx, by, bz);
+      const uint32_t va = voxel_at(p, ta, *Jp, ax, ay, az);
+      const uint32_t vb = voxel_at(p, tb, *Jp, bx, by, bz);
+      const int la = label_of(va, *Jp), lb = label_of(vb, *Jp);
       const float mua = mu_at(p, la, *Jp), mub = mu_at(p, lb, *Jp);
       if (ta >= p.t1) {
         code = tally(p, *Jp);
