@@ -20,7 +20,8 @@
 //     quirk Q9) and the first step starts at the box entry;
 //   * primary and scatter counts are tallied with FP64 atomics.  Every tally
 //     adds 1.0, so sums are exact integers and independent of order;
-//   * statistics are counted per wave from ballots in scalar registers.
+//   * statistics are counted per lane where the events happen and summed over
+//     the wave when a job's counters are flushed.
 #include "internal.h"
 
 namespace monte {
@@ -196,18 +197,36 @@ struct WaveTables {
 };
 
 enum { kDead = 0, kFly = 1, kCollide = 2 };
-enum { kTallyMiss = 0, kTallyPrimary = 1, kTallyScatter = 2 };
+// Per-lane statistics, counted in the branches where the events happen and
+// summed over the wave when a job's counters are flushed.  (Kept as separate
+// scalars: as one struct the compiler left them in scratch memory.)
+enum { kStatHistories, kStatSteps, kStatInteractions, kStatPhoto, kStatCoherent, kStatCompton, kStatTp, kStatTs,
+       kStatMissed, kStatCapped, kStats };
+
+__device__ inline unsigned popc(uint64_t m) { return (unsigned)__popcll(m); }
+
+__device__ inline unsigned long long wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
 // Straight line from the current position to the detector plane x' = sdd - sod.
+enum { kTallyMiss = 0, kTallyPrimary = 1, kTallyScatter = 2 };
+
 __device__ inline int tally(const Photon& p, const McJob& J) {
   const float cx = p.cv, cy = p.sv;
   const float xq = p.px * cx + p.py * cy, yq = -p.px * cy + p.py * cx;
   const float ddx = p.dx * cx + p.dy * cy, ddy = -p.dx * cy + p.dy * cx;
-  if (!(ddx > 0.f)) return kTallyMiss;
+  if (!(ddx > 0.f)) {
+    return kTallyMiss;
+  }
   const float tt = ((J.sdd - J.sod) - xq) * __builtin_amdgcn_rcpf(ddx);
   const float yh = fmaf(tt, ddy, yq), zh = fmaf(tt, p.dz, p.pz);
   const float ui = fmaf(J.us, yh, J.un0), vi = fmaf(J.vs, zh, J.vn0);
-  if (!(ui >= 0.f && ui < (float)J.nu && vi >= 0.f && vi < (float)J.nv)) return kTallyMiss;
+  if (!(ui >= 0.f && ui < (float)J.nu && vi >= 0.f && vi < (float)J.nv)) {
+    return kTallyMiss;
+  }
   int iu = (int)floorf(ui), iv = (int)floorf(vi);
   iu = iu > J.nu - 1 ? J.nu - 1 : iu;
   iv = iv > J.nv - 1 ? J.nv - 1 : iv;
@@ -288,31 +307,6 @@ __device__ inline int label_of(uint32_t i, const McJob& J) {
   return min((int)((const __attribute__((address_space(1))) uint8_t*)J.labels)[i], kMaxLabels - 1);
 }
 
-// Wave-uniform statistics (scalar registers), flushed per job.
-struct WaveCounters {
-  uint32_t histories, steps, interactions, photo, coherent, compton, tp, ts, missed, capped;
-};
-
-__device__ inline unsigned popc(uint64_t m) { return (unsigned)__popcll(m); }
-
-__device__ inline void count_tally(WaveCounters& c, bool did, int code) {
-  c.tp += popc(__ballot(did && code == kTallyPrimary));
-  c.ts += popc(__ballot(did && code == kTallyScatter));
-  c.missed += popc(__ballot(did && code == kTallyMiss));
-}
-
-__device__ inline void flush_counters(WaveCounters& c, McCounters* dst) {
-  if (__lane_id() == 0) {
-    using u64 = unsigned long long;
-    atomicAdd(&dst->histories, (u64)c.histories); atomicAdd(&dst->steps, (u64)c.steps);
-    atomicAdd(&dst->interactions, (u64)c.interactions); atomicAdd(&dst->photo, (u64)c.photo);
-    atomicAdd(&dst->coherent, (u64)c.coherent); atomicAdd(&dst->compton, (u64)c.compton);
-    atomicAdd(&dst->tallied_primary, (u64)c.tp); atomicAdd(&dst->tallied_scatter, (u64)c.ts);
-    atomicAdd(&dst->missed, (u64)c.missed); atomicAdd(&dst->capped, (u64)c.capped);
-  }
-  c = WaveCounters{};
-}
-
 // A parked branch runs once at least this many lanes wait for it (or nothing
 // else can run).
 constexpr unsigned kInteractBatch = 16;
@@ -330,7 +324,30 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   int state = kDead;
   Photon p{};
   Stream rs{};
-  WaveCounters cnt{};
+  // (counted by value: counters handed to tally() by reference were left in
+  // scratch memory by the compiler)
+#define MONTE_NOTE_TALLY(expr)               \
+  do {                                       \
+    const int code_ = (expr);                \
+    n_tp += code_ == kTallyPrimary ? 1u : 0u; \
+    n_ts += code_ == kTallyScatter ? 1u : 0u; \
+    n_miss += code_ == kTallyMiss ? 1u : 0u;  \
+  } while (0)
+  uint32_t n_hist = 0, n_steps = 0, n_int = 0, n_photo = 0, n_coh = 0, n_comp = 0, n_tp = 0, n_ts = 0, n_miss = 0,
+           n_cap = 0;
+  auto flush = [&](McCounters* dst) {
+    const unsigned long long v[kStats] = {wave_sum(n_hist), wave_sum(n_steps), wave_sum(n_int), wave_sum(n_photo),
+                                          wave_sum(n_coh),  wave_sum(n_comp),  wave_sum(n_tp),  wave_sum(n_ts),
+                                          wave_sum(n_miss), wave_sum(n_cap)};
+    if (__lane_id() == 0) {
+      atomicAdd(&dst->histories, v[kStatHistories]); atomicAdd(&dst->steps, v[kStatSteps]);
+      atomicAdd(&dst->interactions, v[kStatInteractions]); atomicAdd(&dst->photo, v[kStatPhoto]);
+      atomicAdd(&dst->coherent, v[kStatCoherent]); atomicAdd(&dst->compton, v[kStatCompton]);
+      atomicAdd(&dst->tallied_primary, v[kStatTp]); atomicAdd(&dst->tallied_scatter, v[kStatTs]);
+      atomicAdd(&dst->missed, v[kStatMissed]); atomicAdd(&dst->capped, v[kStatCapped]);
+    }
+    n_hist = n_steps = n_int = n_photo = n_coh = n_comp = n_tp = n_ts = n_miss = n_cap = 0;
+  };
   const int lane = __lane_id();
   // The job descriptors do not change during the launch: read them through the
   // constant address space, so wave-uniform fields come in by scalar loads.
@@ -359,7 +376,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         if (pj == job || all_dead) {
           const bool switched = pj != job;
           if (switched) {
-            if (job >= 0) flush_counters(cnt, counters + job);
+            if (job >= 0) flush(counters + job);
             job = pj;
           }
           // units run views fastest: the units in flight share photon batches, hence
@@ -386,12 +403,10 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
     if (dead != 0 && next < end && (popc(dead) >= kRefillBatch || fly == 0)) {
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
-      bool born = false, tallied = false;
-      int code = 0;
       if (state == kDead) {
         const uint32_t id = next + rank;
         if (id < end) {
-          born = true;
+          ++n_hist;
           rs.ctr = 0; rs.photon = id; rs.view = (uint32_t)unit_view; rs.left = 0;
           // aim at the centre of pixel id mod (nu*nv) (CBCT_real325.cu:464-466)
           const uint32_t npix = (uint32_t)(Jp->nu * Jp->nv);
@@ -425,41 +440,37 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           if (start_flight(p, *Jp, wt)) {
             state = kFly;
           } else {
-            code = tally(p, *Jp);
-            tallied = true;
+            MONTE_NOTE_TALLY(tally(p, *Jp));
           }
         }
       }
-      cnt.histories += popc(__ballot(born));
-      count_tally(cnt, tallied, code);
       next = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(end, next + popc(dead)));
     }
 
     // 3) interactions of the parked colliding lanes
     if (coll != 0 && (popc(coll) >= kInteractBatch || fly == 0)) {
-      bool is_photo = false, is_coh = false, is_comp = false, capped = false, tallied = false;
-      int code = 0;
       if (state == kCollide && p.nint >= Jp->max_int) {
-        capped = true;
+        ++n_cap;
         state = kDead;
       }
       if (state == kCollide) {
         const float4 m = Jp->tables->m[p.lab][p.eb];
         rs.left = 0;  // interactions start on a fresh block
         const float u3 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+        ++n_int;  // photoelectric, coherent or Compton
         if (u3 <= m.y) {
-          is_photo = true;
+          ++n_photo;
           state = kDead;
         } else {
           if (u3 <= m.z) {
-            is_coh = true;
+            ++n_coh;
             if (Jp->coherent) {
               const float cth = thomson_mu(next_u01(rs, Jp->job_id, Jp->key0, Jp->key1));
               const float phi = 6.283185307179586f * next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
               rotate_dir(p.dx, p.dy, p.dz, cth, phi);
             }
           } else {
-            is_comp = true;
+            ++n_comp;
             const float ro = kahn(p.e, rs, Jp->job_id, Jp->key0, Jp->key1);
             const float lam = 511.0f / p.e, lamd = ro * lam;
             float cth = 1.0f - (lamd - lam);
@@ -473,28 +484,19 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           if (start_flight(p, *Jp, wt)) {
             state = kFly;
           } else {
-            code = tally(p, *Jp);
-            tallied = true;
+            MONTE_NOTE_TALLY(tally(p, *Jp));
             state = kDead;
           }
         }
       }
-      cnt.capped += popc(__ballot(capped));
-      cnt.photo += popc(__ballot(is_photo));
-      cnt.coherent += popc(__ballot(is_coh));
-      cnt.compton += popc(__ballot(is_comp));
-      cnt.interactions += popc(__ballot(is_photo || is_coh || is_comp));
-      count_tally(cnt, tallied, code);
     }
 
     // 4) two Woodcock steps for every flying lane from one Philox block
     //    (distance, accept, distance, accept); both label loads are issued
     //    before either is used, and a flight ending on the first step leaves
     //    the block's other two uniforms unused.
-    bool stepped = false, second = false, tallied = false;
-    int code = 0;
     if (state == kFly) {
-      stepped = true;
+      ++n_steps;
       uint32_t c0 = rs.ctr, c1 = rs.photon, c2 = rs.view, c3 = Jp->job_id;
       philox10(c0, c1, c2, c3, Jp->key0, Jp->key1);
       rs.ctr += 1;
@@ -508,18 +510,16 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       const int la = label_of(va, *Jp), lb = label_of(vb, *Jp);
       const float mua = mu_at(p, la, *Jp), mub = mu_at(p, lb, *Jp);
       if (ta >= p.t1) {
-        code = tally(p, *Jp);
-        tallied = true;
+        MONTE_NOTE_TALLY(tally(p, *Jp));
         state = kDead;
       } else if (u01(c1) * p.mumax < mua) {  // real collision: park until the wave runs interactions
         p.px = ax; p.py = ay; p.pz = az;
         p.lab = la;
         state = kCollide;
       } else {
-        second = true;
+        ++n_steps;
         if (tb >= p.t1) {
-          code = tally(p, *Jp);
-          tallied = true;
+          MONTE_NOTE_TALLY(tally(p, *Jp));
           state = kDead;
         } else if (u01(c3) * p.mumax < mub) {
           p.px = bx; p.py = by; p.pz = bz;
@@ -530,10 +530,9 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         }
       }
     }
-    cnt.steps += popc(__ballot(stepped)) + popc(__ballot(second));
-    count_tally(cnt, tallied, code);
   }
-  if (job >= 0) flush_counters(cnt, counters + job);
+  if (job >= 0) flush(counters + job);
+#undef MONTE_NOTE_TALLY
 }
 
 __global__ void mc_reset_kernel(unsigned int* queue, McCounters* counters, int n_jobs) {
