@@ -819,7 +819,9 @@ __device__ inline void pass_updates_pair2(float* __restrict__ strip, const float
 }
 
 // One pass of a column: lane l takes the C voxels of chunk ch0 + l.  `cell0`
-// is the LDS index of (row 0, left tap); `trash` the lane's trash floats.
+// is the LDS index of (row 0, left tap); `trash` the lane's trash floats;
+// `phases` is the number of interleaved lane phases, or 0 for the folded
+// two-voxel updates (C = 2 and a v-slope in (0.505, 0.999)).
 template <int C>
 __device__ inline void project_pass(float* __restrict__ strip, const float (&v)[C], const ColumnCore& cc,
                                     const ColumnFx& fx, int cell0, int trash, int ch0, int phases,
@@ -827,7 +829,7 @@ __device__ inline void project_pass(float* __restrict__ strip, const float (&v)[
   const int z0 = (ch0 + lane) * C;
   const long long s0 = fx.a + (long long)z0 * fx.d;
   if constexpr (C == 2) {
-    if (phases == 1 && cc.dsv > 0.f && cc.dsv < 0.999f) {
+    if (phases == 0) {
       pass_updates_pair(strip, v, cc, s0, fx.d, fx.lo, fx.hi, z0, cell0, trash, g);
       return;
     }
@@ -848,6 +850,23 @@ struct ColumnBatch {
   float fu = 0.f, w = 0.f, dsv = 0.f;
   double kv = 0.0;
 };
+
+// The projector's per-column constants, computed by lane i for the i-th
+// listed column of a batch (all 64 columns at once), then read by the whole
+// wave with readlane when the column's turn comes.
+struct ColumnPrep {
+  int x = 0, y = 0;
+  float fu = 0.f, w = 0.f;
+  long long a = 0, d = 0;  // sv(z) = a + z d in 32.32 fixed point
+  int phases = 1;          // lane phases, 0 = folded two-voxel updates
+  int cell0 = 0;           // LDS index of (padded row 0, left tap)
+};
+
+__device__ inline long long bcast_ll(long long v, int src) {
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), src);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), src);
+  return ((long long)hi << 32) | (unsigned int)lo;
+}
 
 __device__ inline int permute_i(int dst, int v) { return __builtin_amdgcn_ds_permute(dst << 2, v); }
 __device__ inline float permute_f(int dst, float v) {
@@ -978,29 +997,28 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
   const size_t cell = (size_t)k * n_strips + st;
   const uint32_t beg = offs[cell], end = offs[cell + 1];
 
-  ColumnBatch cb;
+  ColumnPrep cp;
+  int ncols = 0;
+  const ColumnFx win = {0, 0, (long long)g.lo_v * 4294967296LL, (long long)g.hi_v * 4294967296LL};
   // The volume values of the next kAhead columns are in flight while one is
   // accumulated: a column's accumulation takes less time than a load from HBM.
   auto fetch = [&](ColumnVals<C>& dst, int i) {
-    const int x = bcast_i(cb.x, i), y = bcast_i(cb.y, i);
+    const int x = bcast_i(cp.x, i), y = bcast_i(cp.y, i);
     load_column<C, MASK>(dst, volT + ((size_t)y * g.nx + x) * g.nz, x, y, g, lane, npass);
   };
   auto accumulate = [&](const ColumnVals<C>& vals, int i) {
     ColumnCore cc;
-    cc.iu = bcast_i(cb.iu, i);
-    cc.fu = bcast_f(cb.fu, i);
-    cc.w = bcast_f(cb.w, i);
+    cc.fu = bcast_f(cp.fu, i);
+    cc.w = bcast_f(cp.w, i);
     cc.ok = true;
-    cc.kv = bcast_d(cb.kv, i);
-    cc.dsv = bcast_f(cb.dsv, i);
-    const float step = (float)C * fabsf(cc.dsv);
-    const int phases = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
-    const int t0 = cc.iu + kPadLo - pu0;  // strip column of the left tap: 0 .. kStripW - 1
-    const int cell0 = (kPadLo - lds_r0) * kStripP + t0 + kStripC0;
-    const ColumnFx fx = column_fx(cc, c, g);
+    const int phases = bcast_i(cp.phases, i);
+    const int cell0 = bcast_i(cp.cell0, i);
+    ColumnFx fx = win;
+    fx.a = bcast_ll(cp.a, i);
+    fx.d = bcast_ll(cp.d, i);
     if constexpr (C == 2) {
-      // three pair passes: the first and last together, then the middle one
-      if (npass == 3 && phases == 1 && cc.dsv > 0.f && cc.dsv < 0.999f) {
+      // three folded passes: the first and last together, then the middle one
+      if (npass == 3 && phases == 0) {
         const int za = 2 * lane, zb = 2 * (128 + lane);
         pass_updates_pair2(strip, vals.v[0], vals.v[2], cc, fx.a + (long long)za * fx.d,
                            fx.a + (long long)zb * fx.d, fx.d, fx.lo, fx.hi, za, zb, cell0, trash + 2 * lane, g);
@@ -1012,7 +1030,7 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
     for (int p = 0; p < NP; ++p)
       if (p < npass) project_pass<C>(strip, vals.v[p], cc, fx, cell0, trash + 2 * lane, p * 64, phases, g, lane);
     if (npass > NP) {  // columns longer than the prefetch window load the rest in place
-      const int x = bcast_i(cb.x, i), y = bcast_i(cb.y, i);
+      const int x = bcast_i(cp.x, i), y = bcast_i(cp.y, i);
       const float* col = volT + ((size_t)y * g.nx + x) * g.nz;
       for (int p = NP; p < npass; ++p) {
         float v[C];
@@ -1021,33 +1039,37 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
       }
     }
   };
-  // Batches of 64 listed columns, lane i holding the i-th with the same column
+  // Batches of 64 listed columns: lane i prepares the i-th with the same column
   // arithmetic the enumeration used to select it.
   for (uint32_t b0 = beg; b0 < end; b0 += 64) {
-    cb.n = (int)min(64u, end - b0);
-    if (lane < cb.n) {
+    ncols = (int)min(64u, end - b0);
+    if (lane < ncols) {
       const uint32_t e = list[b0 + lane];
-      cb.x = (int)(e & 0xffffu);
-      cb.y = (int)(e >> 16);
-      const double X = voxel_x(g, cb.x), Y = voxel_y(g, cb.y);
+      cp.x = (int)(e & 0xffffu);
+      cp.y = (int)(e >> 16);
+      const double X = voxel_x(g, cp.x), Y = voxel_y(g, cp.y);
       const ColumnCore mine = column_core(c, X, Y, (float)X, (float)Y, g);
-      cb.iu = mine.iu;
-      cb.fu = mine.fu;
-      cb.w = mine.w;
-      cb.dsv = mine.dsv;
-      cb.kv = mine.kv;
+      const ColumnFx f = column_fx(mine, c, g);
+      cp.fu = mine.fu;
+      cp.w = mine.w;
+      cp.a = f.a;
+      cp.d = f.d;
+      const float step = (float)C * fabsf(mine.dsv);
+      const int ph = step > 1.01f ? 1 : min(64, (int)ceilf(1.01f / fmaxf(step, 1e-6f)));
+      cp.phases = (C == 2 && ph == 1 && mine.dsv > 0.f && mine.dsv < 0.999f) ? 0 : ph;
+      cp.cell0 = (kPadLo - lds_r0) * kStripP + (mine.iu + kPadLo - pu0) + kStripC0;
     }
     ColumnVals<C> ring[kAhead];
 #pragma unroll
     for (int q = 0; q < kAhead; ++q)
-      if (q < cb.n) fetch(ring[q], q);
-    for (int i0 = 0; i0 < cb.n; i0 += kAhead) {
+      if (q < ncols) fetch(ring[q], q);
+    for (int i0 = 0; i0 < ncols; i0 += kAhead) {
 #pragma unroll
       for (int q = 0; q < kAhead; ++q) {
         const int i = i0 + q;
-        if (i < cb.n) {
+        if (i < ncols) {
           const ColumnVals<C> vals = ring[q];
-          if (i + kAhead < cb.n) fetch(ring[q], i + kAhead);
+          if (i + kAhead < ncols) fetch(ring[q], i + kAhead);
           accumulate(vals, i);
         }
       }
