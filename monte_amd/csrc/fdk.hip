@@ -90,8 +90,9 @@ __global__ __launch_bounds__(256) void filter_fwd_mfma_kernel(const float* __res
                                                               float* __restrict__ out,
                                                               const float* __restrict__ taps, FilterParams fp) {
   extern __shared__ float sm[];
-  float* As = sm;  // [kFkc][kFas]
-  float* Ht = sm + kFkc * kFas;
+  float* const A0 = sm;  // double-buffered [kFkc][kFas]
+  float* const A1 = sm + kFkc * kFas;
+  float* Ht = sm + 2 * kFkc * kFas;
   const int nu = fp.nu, nv = fp.nv;
   const int v0 = blockIdx.x * kFbm, b0 = blockIdx.y * kFbn, view = blockIdx.z;
   stage_taps(Ht, taps, nu);
@@ -106,18 +107,27 @@ __global__ __launch_bounds__(256) void filter_fwd_mfma_kernel(const float* __res
       pre[j] = src[(size_t)(c < nu ? c : nu - 1) * nv + vc];
     }
   };
-  load(0);
-  floatx16 acc[2] = {};
-  for (int c0 = 0; c0 < nu; c0 += kFkc) {
-    __syncthreads();
+  auto stage = [&](float* A, int c0) {
 #pragma unroll
     for (int j = 0; j < kFkc / 4; ++j) {
       const int kc = tc + 4 * j, c = c0 + kc;
-      As[kc * kFas + tv] = (c < nu && v < nv) ? pre[j] * cos_weight(fp, c, v) : 0.f;
+      A[kc * kFas + tv] = (c < nu && v < nv) ? pre[j] * cos_weight(fp, c, v) : 0.f;
     }
+  };
+  load(0);
+  stage(A0, 0);
+  __syncthreads();
+  floatx16 acc[2] = {};
+  // chunk k is multiplied from one buffer while chunk k + 1 is loaded and
+  // staged into the other; one barrier per chunk
+  int buf = 0;
+  for (int c0 = 0; c0 < nu; c0 += kFkc) {
+    const bool more = c0 + kFkc < nu;
+    if (more) load(c0 + kFkc);
+    mfma_chunk(buf ? A1 : A0, Ht, c0, b0, nu, acc);
+    if (more) stage(buf ? A0 : A1, c0 + kFkc);
     __syncthreads();
-    if (c0 + kFkc < nu) load(c0 + kFkc);
-    mfma_chunk(As, Ht, c0, b0, nu, acc);
+    buf ^= 1;
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float* dst = out + (size_t)view * fp.rows * fp.pitch;
@@ -139,8 +149,9 @@ __global__ __launch_bounds__(256) void filter_fwd_mfma_kernel(const float* __res
 __global__ __launch_bounds__(256) void filter_T_mfma_kernel(const float* __restrict__ in, float* __restrict__ proj,
                                                             const float* __restrict__ taps, FilterParams fp) {
   extern __shared__ float sm[];
-  float* As = sm;
-  float* Ht = sm + kFkc * kFas;
+  float* const A0 = sm;  // double-buffered [kFkc][kFas]
+  float* const A1 = sm + kFkc * kFas;
+  float* Ht = sm + 2 * kFkc * kFas;
   const int nu = fp.nu, nv = fp.nv;
   const int v0 = blockIdx.x * kFbm, b0 = blockIdx.y * kFbn, view = blockIdx.z;
   stage_taps(Ht, taps, nu);
@@ -155,22 +166,29 @@ __global__ __launch_bounds__(256) void filter_T_mfma_kernel(const float* __restr
       pre[j] = src[(size_t)((v < nv ? v : nv - 1) + kPadLo) * fp.pitch + cc + kPadLo];
     }
   };
-  load(0);
-  floatx16 acc[2] = {};
-  for (int c0 = 0; c0 < nu; c0 += kFkc) {
-    __syncthreads();
+  auto stage = [&](float* A, int c0) {
     const bool cin = c0 + tk < nu;
 #pragma unroll
     for (int j = 0; j < kFbm / 8; ++j) {
       const int vl = tr + 8 * j;
-      As[tk * kFas + vl] = (cin && v0 + vl < nv) ? pre[j] : 0.f;
+      A[tk * kFas + vl] = (cin && v0 + vl < nv) ? pre[j] : 0.f;
     }
-    __syncthreads();
-    if (c0 + kFkc < nu) load(c0 + kFkc);
-    mfma_chunk(As, Ht, c0, b0, nu, acc);
-  }
-  // stage D as Ds[b_local][v_local] over the same LDS, then write proj[b][v] along v
+  };
+  load(0);
+  stage(A0, 0);
   __syncthreads();
+  floatx16 acc[2] = {};
+  int buf = 0;
+  for (int c0 = 0; c0 < nu; c0 += kFkc) {
+    const bool more = c0 + kFkc < nu;
+    if (more) load(c0 + kFkc);
+    mfma_chunk(buf ? A1 : A0, Ht, c0, b0, nu, acc);
+    if (more) stage(buf ? A0 : A1, c0 + kFkc);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // stage D as Ds[b_local][v_local] over the same LDS (the loop ended on a
+  // barrier), then write proj[b][v] along v
   float* Ds = sm;  // [kFbn][kFas]
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -191,10 +209,10 @@ __global__ __launch_bounds__(256) void filter_T_mfma_kernel(const float* __restr
   }
 }
 
-// LDS bytes: As and the taps; the transposed filter also stages its 128 x 64
+// LDS bytes: two As buffers and the taps; the transposed filter also stages its 128 x 64
 // result over the same memory.
 static size_t filter_mfma_lds(int nu, bool transposed) {
-  const size_t stage = (size_t)kFkc * kFas + 2 * (size_t)nu + kFkc + 2 * kFbn;
+  const size_t stage = 2 * (size_t)kFkc * kFas + 2 * (size_t)nu + kFkc + 2 * kFbn;
   const size_t out = transposed ? (size_t)kFbn * kFas : 0;
   return sizeof(float) * (stage > out ? stage : out);
 }
@@ -477,8 +495,8 @@ __global__ __launch_bounds__(256) void backproject_kernel(const float4* __restri
       float fv;
       const bool okv = chunk_tap(cc, cb, joff + j, off0, prow, off, fv);
       const float4 t = *(const float4*)(vb + off);
-      const float a = fmaf(cc.fu, t.y - t.x, t.x);
-      const float b = fmaf(cc.fu, t.w - t.z, t.z);
+      const float a = fmaf(cc.fu, t.y, t.x);
+      const float b = fmaf(cc.fu, t.w, t.z);
       const float val = fmaf(fv, b - a, a);
       acc[j] = fmaf((okv && live[j]) ? wu : 0.f, val, acc[j]);
     }
@@ -502,8 +520,10 @@ static void bp_tiling(const BpGrid& grid, const monte_tuning& t, int zc, int& ti
   nblocks = (long)tiles_x * tiles_y * zchunks;
 }
 
-// quads [view][rows][pitch]: quad (r, c) = (p[r][c], p[r][c + 1], p[r + 1][c], p[r + 1][c + 1])
-// of the padded filtered view p, zero past the last row or column.
+// quads [view][rows][pitch]: quad (r, c) = (p[r][c], p[r][c + 1] - p[r][c], p[r + 1][c],
+// p[r + 1][c + 1] - p[r + 1][c]) of the padded filtered view p (zero past the last
+// row or column): the two rows of a voxel's bilinear taps with their u
+// differences, which the backprojector would otherwise form per voxel.
 __global__ void tap_quads_kernel(const float* __restrict__ pad, float4* __restrict__ quads, int rows, int pitch,
                                  size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -513,12 +533,9 @@ __global__ void tap_quads_kernel(const float* __restrict__ pad, float4* __restri
     const int r = rc / pitch, cl = rc - r * pitch;
     const float* p = pad + i;
     const bool rr = r + 1 < rows, cr = cl + 1 < pitch;
-    float4 q;
-    q.x = p[0];
-    q.y = cr ? p[1] : 0.f;
-    q.z = rr ? p[pitch] : 0.f;
-    q.w = (rr && cr) ? p[pitch + 1] : 0.f;
-    quads[i] = q;
+    const float p00 = p[0], p01 = cr ? p[1] : 0.f;
+    const float p10 = rr ? p[pitch] : 0.f, p11 = (rr && cr) ? p[pitch + 1] : 0.f;
+    quads[i] = make_float4(p00, p01 - p00, p10, p11 - p10);
   }
 }
 

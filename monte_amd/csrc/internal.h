@@ -192,7 +192,8 @@ hipError_t launch_unpad_copy(const float* filt_pad, float* filt, int n_views, in
 hipError_t launch_fill_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, int wrap, hipStream_t s);
 hipError_t launch_fold_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, hipStream_t s);
 // Tap quads of the padded filtered views: quads[i] holds the 2 x 2 bilinear
-// neighbourhood whose top-left cell is padded cell i (n_views * rows * pitch).
+// neighbourhood whose top-left cell is padded cell i (n_views * rows * pitch),
+// as (p00, p01 - p00, p10, p11 - p10).
 hipError_t launch_tap_quads(const float* filt_pad, float4* quads, int n_views, int rows, int pitch, hipStream_t s);
 hipError_t launch_backproject(const float4* quads, const BpView* views, int view0, int nviews_chunk,
                               const BpGrid& grid, float* vol, int accumulate, const monte_tuning& t, hipStream_t s);
