@@ -1,6 +1,7 @@
 // C ABI of libmonte.so (declared in include/monte.h): argument checking,
 // workspaces, constant caching, host-pointer staging, MC job construction,
 // autotuning and hipGraph capture.  All device work is in fdk.hip and mc.hip.
+#include <climits>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -255,6 +256,7 @@ int run_filter_T_padded(monte_ctx* c, const monte_geom& g, const float* pad, flo
 // ------------------------------------------------------------------ MC
 struct HostJob {
   McJob d{};
+  double org[3] = {0, 0, 0}, vox[3] = {1, 1, 1};  // phantom grid origin (corner) and voxel size
   McTables tables{};
   std::vector<float> cdf;       // empty = mono
   std::vector<float2> view_cs;  // all views of the geometry
@@ -339,6 +341,9 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
   J.view_begin = vb;
   J.n_views_sim = ve - vb;
   J.vx = (float)ph->vx; J.vy = (float)ph->vy; J.vz = (float)ph->vz;
+  hj.org[0] = ph->ox - 0.5 * ph->nx * ph->vx; hj.org[1] = ph->oy - 0.5 * ph->ny * ph->vy;
+  hj.org[2] = ph->oz - 0.5 * ph->nz * ph->vz;
+  hj.vox[0] = ph->vx; hj.vox[1] = ph->vy; hj.vox[2] = ph->vz;
   J.bx0 = (float)(ph->ox - 0.5 * ph->nx * ph->vx); J.bx1 = (float)(ph->ox + 0.5 * ph->nx * ph->vx);
   J.by0 = (float)(ph->oy - 0.5 * ph->ny * ph->vy); J.by1 = (float)(ph->oy + 0.5 * ph->ny * ph->vy);
   J.bz0 = (float)(ph->oz - 0.5 * ph->nz * ph->vz); J.bz1 = (float)(ph->oz + 0.5 * ph->nz * ph->vz);
@@ -380,8 +385,43 @@ int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blo
   }
   const size_t q_off = off; off = align_up(off + 256, 256);
   const size_t cnt_off = off; off = align_up(off + sizeof(McCounters) * n, 256);
+  const size_t box_off = off; off = align_up(off + sizeof(int) * 8 * n, 256);
   char* base = (char*)workspace(c, kSlotMc, off);
   if (!base) return fail(c, MONTE_ERR_NOMEM, "MC descriptor allocation failed");
+
+  // Flights are clipped to the bounding box of the voxels that attenuate at
+  // some energy: outside it mu is 0, so no history changes, and no Woodcock
+  // steps are spent crossing the empty margin of the phantom grid.
+  {
+    std::vector<int> box(8 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      int* b = box.data() + 8 * i;
+      b[0] = b[1] = b[2] = INT_MAX;
+      b[3] = b[4] = b[5] = -1;
+    }
+    int* dbox = (int*)(base + box_off);
+    MONTE_HIP(c, hipMemcpyAsync(dbox, box.data(), sizeof(int) * box.size(), hipMemcpyHostToDevice, c->stream));
+    for (int i = 0; i < n; ++i) {
+      uint32_t solid = 0;
+      for (int l = 0; l < kMaxLabels; ++l)
+        for (int e = 0; e < kRows; ++e)
+          if (hj[i].tables.m[l][e].x > 0.f) { solid |= 1u << l; break; }
+      const McJob& J = hj[i].d;
+      MONTE_HIP(c, launch_support_box(J.labels, J.nx, J.ny, J.nz, solid, dbox + 8 * i, c->stream));
+    }
+    MONTE_HIP(c, hipMemcpyAsync(box.data(), dbox, sizeof(int) * box.size(), hipMemcpyDeviceToHost, c->stream));
+    MONTE_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+      const int* b = box.data() + 8 * i;
+      if (b[3] < 0) continue;  // nothing attenuates: keep the grid box
+      McJob& J = hj[i].d;
+      const double* o = hj[i].org;
+      const double* v = hj[i].vox;
+      J.bx0 = (float)(o[0] + b[0] * v[0]); J.bx1 = (float)(o[0] + (b[3] + 1) * v[0]);
+      J.by0 = (float)(o[1] + b[1] * v[1]); J.by1 = (float)(o[1] + (b[4] + 1) * v[1]);
+      J.bz0 = (float)(o[2] + b[2] * v[2]); J.bz1 = (float)(o[2] + (b[5] + 1) * v[2]);
+    }
+  }
 
   uint64_t units = 0;
   std::vector<McJob> jobs(n);

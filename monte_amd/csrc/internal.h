@@ -223,6 +223,9 @@ hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* 
 hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, unsigned int* queue,
                                McCounters* counters, int batch, int block, int grid, hipStream_t s);
 hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs, hipStream_t s);
+// Bounding box (voxel indices, inclusive) of voxels whose clamped label has bit set in `solid`:
+// box[0..2] = minima, box[3..5] = maxima; the caller initialises it to {INT_MAX x 3, -1 x 3}.
+hipError_t launch_support_box(const uint8_t* labels, int nx, int ny, int nz, uint32_t solid, int* box, hipStream_t s);
 hipError_t launch_primary_exact(const PrimaryGeom& P, const double* cs64, const uint8_t* labels,
                                 const McTables* tables, float e_kev, const float* spec_w, int use_spec, float* out,
                                 hipStream_t s);

@@ -22,6 +22,9 @@
 //     adds 1.0, so sums are exact integers and independent of order;
 //   * statistics are counted per lane where the events happen and summed over
 //     the wave when a job's counters are flushed.
+#include <algorithm>
+#include <climits>
+
 #include "internal.h"
 
 namespace monte {
@@ -539,6 +542,58 @@ __global__ void mc_reset_kernel(unsigned int* queue, McCounters* counters, int n
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *queue = 0u;
   if (i < n_jobs) counters[i] = McCounters{};
+}
+
+// Bounding box of the voxels whose label attenuates at some energy
+// (`solid` has bit l set for such labels; labels are clamped as in transport):
+// box = {x_min, y_min, z_min, x_max, y_max, z_max}, initialised by the caller
+// to {INT_MAX x 3, -1 x 3}.  One block per z slice, reduced in the block, so
+// the box takes six atomics per slice.
+__global__ __launch_bounds__(256) void support_box_kernel(const uint8_t* __restrict__ labels, int nx, int ny, int nz,
+                                                          uint32_t solid, int* __restrict__ box) {
+  __shared__ int red[4][4];
+  const int z = blockIdx.x;
+  const uint8_t* slice = labels + (size_t)z * ny * nx;
+  int xmin = INT_MAX, xmax = -1, ymin = INT_MAX, ymax = -1;
+  for (int y = 0; y < ny; ++y) {
+    for (int x = threadIdx.x; x < nx; x += 256) {
+      const int lab = min((int)slice[(size_t)y * nx + x], kMaxLabels - 1);
+      if ((solid >> lab) & 1u) {
+        xmin = min(xmin, x);
+        xmax = max(xmax, x);
+        ymin = min(ymin, y);
+        ymax = y;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    xmin = min(xmin, __shfl_xor(xmin, o, 64));
+    xmax = max(xmax, __shfl_xor(xmax, o, 64));
+    ymin = min(ymin, __shfl_xor(ymin, o, 64));
+    ymax = max(ymax, __shfl_xor(ymax, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w][0] = xmin; red[w][1] = xmax; red[w][2] = ymin; red[w][3] = ymax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 4; ++i) {
+      xmin = min(xmin, red[i][0]); xmax = max(xmax, red[i][1]);
+      ymin = min(ymin, red[i][2]); ymax = max(ymax, red[i][3]);
+    }
+    if (xmax >= 0) {
+      atomicMin(box + 0, xmin); atomicMin(box + 1, ymin); atomicMin(box + 2, z);
+      atomicMax(box + 3, xmax); atomicMax(box + 4, ymax); atomicMax(box + 5, z);
+    }
+  }
+}
+
+hipError_t launch_support_box(const uint8_t* labels, int nx, int ny, int nz, uint32_t solid, int* box, hipStream_t s) {
+  if (nx < 1 || ny < 1 || nz < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(support_box_kernel, dim3(nz), dim3(256), 0, s, labels, nx, ny, nz, solid, box);
+  return hipGetLastError();
 }
 
 hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs, hipStream_t s) {

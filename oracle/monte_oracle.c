@@ -684,7 +684,44 @@ typedef struct {
   orc_mc_stats *stats;
   pthread_mutex_t mu;
   long chunks_per_view;
+  double c0[3], c1[3]; /* flight box: bounding box of the attenuating voxels */
 } orc_mc_ctx;
+
+/* Bounding box of the voxels whose label attenuates at some energy (labels
+ * clamped to the table as in transport).  Outside it mu is 0, so clipping
+ * flights to it changes no history's outcome, only the Woodcock steps spent
+ * crossing the empty margin; the GPU transport clips to the same box.  With
+ * no attenuating voxel the grid box is kept. */
+static void support_box(const orc_phantom *ph, const orc_mc_tables *mt, double c0[3], double c1[3]) {
+  double b0[3], b1[3];
+  ph_bounds(ph, b0, b1);
+  int solid[ORC_MAX_MAT];
+  for (int l = 0; l < ORC_MAX_MAT; ++l) {
+    solid[l] = 0;
+    for (int e = 0; e < ORC_NROWS; ++e)
+      if (mt->mu[l][e] > 0.f) { solid[l] = 1; break; }
+  }
+  int lo[3] = {ph->nx, ph->ny, ph->nz}, hi[3] = {-1, -1, -1};
+  for (int z = 0; z < ph->nz; ++z)
+    for (int y = 0; y < ph->ny; ++y) {
+      const uint8_t *row = ph->labels + ((size_t)z * ph->ny + y) * ph->nx;
+      for (int x = 0; x < ph->nx; ++x) {
+        int lab = row[x] < ORC_MAX_MAT ? row[x] : ORC_MAX_MAT - 1;
+        if (!solid[lab]) continue;
+        const int idx[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) {
+          if (idx[a] < lo[a]) lo[a] = idx[a];
+          if (idx[a] > hi[a]) hi[a] = idx[a];
+        }
+      }
+    }
+  const double vox[3] = {ph->vx, ph->vy, ph->vz};
+  for (int a = 0; a < 3; ++a) {
+    if (hi[0] < 0) { c0[a] = b0[a]; c1[a] = b1[a]; continue; }
+    c0[a] = b0[a] + lo[a] * vox[a];
+    c1[a] = b0[a] + (hi[a] + 1) * vox[a];
+  }
+}
 
 static void mc_items(void *vctx, long i0, long i1) {
   orc_mc_ctx *c = (orc_mc_ctx *)vctx;
@@ -731,10 +768,11 @@ static void mc_items(void *vctx, long i0, long i1) {
       int eb = ebin_of(E);
       int nint = 0;
       for (;;) {
-        /* flight: Woodcock inside the phantom box, straight line outside */
+        /* flight: Woodcock inside the box of attenuating voxels, straight
+         * line outside */
         double t0, t1;
         int real = 0;
-        if (ray_box(pos, dir, b0, b1, &t0, &t1) && t1 > 0 && mt->mumax[eb] > 0.f) {
+        if (ray_box(pos, dir, c->c0, c->c1, &t0, &t1) && t1 > 0 && mt->mumax[eb] > 0.f) {
           /* advance to the box entry so flight parameters stay small */
           if (t0 > 0) {
             pos[0] += t0 * dir[0]; pos[1] += t0 * dir[1]; pos[2] += t0 * dir[2];
@@ -834,6 +872,7 @@ void orc_mc_run(const orc_geom *g, const orc_phantom *ph, const orc_tables *t, c
   orc_mc_stats local = {0};
   orc_mc_ctx c;
   c.g = g; c.ph = ph; c.cfg = cfg; c.mt = mt;
+  support_box(ph, mt, c.c0, c.c1);
   c.primary = primary; c.scatter = scatter; c.stats = stats ? stats : &local;
   pthread_mutex_init(&c.mu, NULL);
   int nv = cfg->view_end - cfg->view_begin;
