@@ -16,8 +16,10 @@
 //     from its batch (ballot + mbcnt prefix ranks), only once enough lanes
 //     are parked or nothing else can run, so those branches execute near full
 //     width instead of for one lane at a time;
-//   * flights are clipped to the phantom box (no Woodcock steps through air,
-//     quirk Q9) and the first step starts at the box entry;
+//   * flights are clipped to the bounding box of the voxels that attenuate
+//     (the reference clips to the phantom box, quirk Q9; mu is 0 in the
+//     margin between the two, so no history changes) and the first step
+//     starts at the box entry;
 //   * primary and scatter counts are tallied with FP64 atomics.  Every tally
 //     adds 1.0, so sums are exact integers and independent of order;
 //   * statistics are counted per lane where the events happen and summed over
