@@ -49,8 +49,8 @@ def parse():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: every path here runs on a "
+              "single GPU (MC would shard by views, FDK by z-slabs)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

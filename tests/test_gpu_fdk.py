@@ -1,6 +1,6 @@
 """FDK on the GPU against the reference's own output and the FP64 oracle.
 
-Tolerance (BASELINE: FDK volumes within 1e-5 relative of an FP64 oracle):
+Tolerance (FDK volumes within 1e-5 relative of an FP64 oracle):
 max |gpu - ref| / max |ref| <= 1e-5, with the l2-relative error also bounded.
 """
 import os
