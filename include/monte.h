@@ -48,7 +48,7 @@ typedef enum monte_status {
   MONTE_ERR_NOMEM = -3,       /* device allocation failed */
   MONTE_ERR_STATE = -4,       /* prerequisite missing (e.g. tables not loaded) */
   MONTE_ERR_IO = -5,          /* file open / parse failure */
-  MONTE_ERR_UNSUPPORTED = -6  /* valid request this build does not implement */
+  MONTE_ERR_UNSUPPORTED = -6  /* valid request the library does not implement */
 } monte_status;
 
 typedef enum monte_preset {

@@ -4,7 +4,8 @@
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (monte_amd/, include/,
  * libmonte.so) links, loads or calls this library.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only as
- * the checker or as the timed CPU baseline -- never as the thing measured.
+ * the correctness reference or as the timed CPU baseline -- never as the
+ * thing measured.
  *
  * What each part restates (reference paths are relative to the reference
  * repository root):

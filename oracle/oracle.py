@@ -1,7 +1,7 @@
 """ctypes front end of the FP64 CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+bench.py's cpu_baseline leg, as the correctness reference or the timed CPU baseline.  The
 product package (monte_amd) never imports it.
 """
 from __future__ import annotations
