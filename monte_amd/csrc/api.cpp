@@ -480,7 +480,7 @@ void fill_stats(monte_mc_stats* s, const McCounters& k, double ms) {
 monte_tuning default_tuning() {
   monte_tuning t{};
   t.mc_block = 256;
-  t.mc_batch = 64;          // fastest at 325^3 / 512^2 / 360 views x 1e7 photons
+  t.mc_batch = 32;          // with 5 blocks per CU, fastest at 325^3 / 512^2 / 360 views x 1e7 photons
   t.mc_blocks_per_cu = 5;
   t.bp_zchunk = 16;
   t.bp_tile_x = 8;   // fastest at 325^3 / 512^2 / 360 views
@@ -950,7 +950,7 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
     if (!tally) return fail(c, MONTE_ERR_NOMEM, "tuning tallies");
     float best_ms = 1e30f;
     const int blocks[] = {64, 128, 256};
-    const int batches[] = {8, 16, 32, 64};
+    const int batches[] = {16, 32, 48, 64};
     const int bpcs[] = {2, 4, 5, 8};
     for (int blk : blocks)
       for (int bt : batches)
