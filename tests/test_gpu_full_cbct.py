@@ -1,0 +1,135 @@
+"""Size-independent properties at the full CBCT workload: 325^3 labels at 0.5 mm,
+512 x 512 detector at 1.2 mm, SOD 600 / SDD 1200 mm, 360 views.
+
+The oracle cannot run these sizes in test time, so the checks are properties
+that hold at any size:
+
+* FDK and its adjoint: <A x, y> = <x, A^T y> to 1e-5 relative (sums in FP64),
+  both operators deterministic, a batch equal to its single stacks and the
+  hipGraph replay equal to the eager call, bitwise;
+* MC: every history ends exactly once (primary or scatter tally, absorption,
+  detector miss, interaction cap), runs are bitwise reproducible and invariant
+  under the launch shape, and the primary tally is unbiased against the exact
+  Beer-Lambert line integrals (chi-square per degree of freedom near 1).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def full():
+    import torch
+
+    from monte_amd import geometry as G
+    from monte_amd import phantoms
+    from monte_amd.mc import device_labels
+
+    ph = phantoms.full_phantom()
+    return G.full(), ph, device_labels(ph, torch.device("cuda", 0))
+
+
+def _randn(shape, seed):
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    return torch.randn(shape, generator=g, device="cuda", dtype=torch.float32)
+
+
+def test_full_fdk_adjoint_identity(ctx, full):
+    import torch
+
+    from monte_amd.fdk import fdk, fdk_adjoint
+
+    geom, _, _ = full
+    x = _randn(geom.proj_shape, 1)
+    ax = fdk(x, geom, ctx)
+    # y close to A x keeps <A x, y> far from zero, so the relative test is meaningful
+    y = ax + 0.1 * ax.std() * _randn(geom.vol_shape, 2)
+    aty = fdk_adjoint(y, geom, ctx)
+    lhs = float((ax.double() * y.double()).sum())
+    rhs = float((x.double() * aty.double()).sum())
+    print("full <Ax,y>", lhs, "<x,A^T y>", rhs, "rel", abs(lhs - rhs) / abs(lhs), flush=True)
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    assert torch.equal(fdk(x, geom, ctx), ax)
+    assert torch.equal(fdk_adjoint(y, geom, ctx), aty)
+
+
+def test_full_fdk_batch_and_graph(ctx, full):
+    import torch
+
+    from monte_amd.fdk import fdk, fdk_graph
+
+    geom, _, _ = full
+    proj = torch.stack([_randn(geom.proj_shape, 3), _randn(geom.proj_shape, 4)])
+    single = [fdk(proj[b], geom, ctx) for b in range(2)]
+    batch = fdk(proj, geom, ctx)
+    for b in range(2):
+        assert torch.equal(batch[b], single[b]), b
+    out = torch.full_like(batch, 7.0)
+    for rep in range(2):
+        fdk_graph(proj, geom, out, ctx)
+        torch.cuda.synchronize()
+        assert torch.equal(out, batch), rep
+        out.fill_(-1.0)
+        torch.cuda.synchronize()
+
+
+def _ends(st):
+    return st["tallied_primary"] + st["tallied_scatter"] + st["photo"] + st["missed"] + st["capped"]
+
+
+def test_full_mc_conservation_and_reproducibility(ctx, full):
+    import torch
+
+    from monte_amd import tables
+    from monte_amd.mc import McSettings, mc_project
+
+    geom, ph, labels = full
+    npix = geom.nu * geom.nv
+    s = McSettings(photons_per_view=npix, spectrum=tables.kramers_spectrum(120.0, 2.5), seed=7)
+    p0, s0, st0 = mc_project(ph, geom, s, labels_dev=labels, ctx=ctx)
+    print("full MC stats", st0, flush=True)
+    assert st0["histories"] == npix * geom.n_views
+    assert _ends(st0) == st0["histories"]
+    assert st0["tallied_primary"] == int(p0.sum()) and st0["tallied_scatter"] == int(s0.sum())
+    assert st0["interactions"] == st0["photo"] + st0["coherent"] + st0["compton"]
+    p1, s1, _ = mc_project(ph, geom, s, labels_dev=labels, ctx=ctx)
+    assert torch.equal(p1, p0) and torch.equal(s1, s0)
+    saved = ctx.tuning
+    try:
+        ctx.set_tuning(mc_block=128, mc_batch=16, mc_blocks_per_cu=2)
+        p2, s2, st2 = mc_project(ph, geom, s, labels_dev=labels, ctx=ctx)
+    finally:
+        ctx.set_tuning(**saved)
+    assert torch.equal(p2, p0) and torch.equal(s2, s0)
+    assert {k: st2[k] for k in st0 if k != "kernel_ms"} == {k: st0[k] for k in st0 if k != "kernel_ms"}
+
+
+def test_full_mc_primary_unbiased(ctx, full):
+    from monte_amd import geometry as G
+    from monte_amd.mc import McSettings, mc_project, primary_exact
+
+    _, ph, labels = full
+    geom = G.full(n_views=4)
+    n0 = 64
+    N = n0 * geom.nu * geom.nv
+    prim, _, st = mc_project(ph, geom, McSettings(photons_per_view=N, e_kev=60.0, seed=11), labels_dev=labels,
+                             ctx=ctx)
+    assert _ends(st) == st["histories"] == N * geom.n_views
+    L = primary_exact(ph, geom, e_kev=60.0, labels_dev=labels, ctx=ctx).double().cpu().numpy()
+    a = prim.cpu().numpy()
+    p = np.exp(-L)
+    inside = (p < 0.999) & (p > 1e-4)
+    mean = n0 * p[inside]
+    var = n0 * p[inside] * (1 - p[inside])
+    z = (a[inside] - mean) / np.sqrt(var)
+    chi2 = float(np.mean(z ** 2))
+    bias = float((a[inside] - mean).sum() / np.sqrt(var.sum()))
+    print(f"pixels {inside.sum()}  chi2/dof {chi2:.4f}  total bias {bias:.2f} sigma", flush=True)
+    assert inside.sum() > 100_000
+    assert 0.97 < chi2 < 1.03
+    assert abs(bias) < 5.0
+    # photons through air only are never absorbed
+    assert np.all(a[L == 0] == n0)
