@@ -44,14 +44,16 @@ __device__ inline float cos_weight(const FilterParams& fp, int r, int c) {
 //   out[v][b] = cs sum_c A[v][c] H[c][b],  H[c][b] = h[c - b + nu - 1],
 // with A the cosine-weighted projection (forward) or the filtered stack
 // (transpose; h is symmetric, so the same H serves).  Each block computes a
-// 64 (v) x 128 (b) tile with v_mfma_f32_32x32x2_f32: wave w owns rows
-// 32 (w & 1) and the two 32-column tiles at 64 (w >> 1).  A is staged in LDS
+// 64 (v) x 64 kFnt (b) tile with v_mfma_f32_32x32x2_f32: wave w owns rows
+// 32 (w & 1) and the kFnt 32-column tiles at 32 kFnt (w >> 1).  A is staged in LDS
 // 32 columns of c at a time as As[c][v]; H is read straight from the taps in
 // LDS, shifted by kFbn so every index of the tile stays inside the array.
 // Operand layouts (one wave, 32 x 32 x 2): A from lane l is A[l % 32][l / 32],
 // B from lane l is B[l / 32][l % 32], and accumulator i of lane l is
 // D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
-constexpr int kFbm = 64, kFbn = 128, kFkc = 32;
+constexpr int kFnt = 4;  // 32-column product tiles per wave
+constexpr int kFbm = 64, kFbn = 64 * kFnt, kFkc = 32;
+static_assert(kFnt % 2 == 0, "the transposed filter stages its result two tiles per wave at a time");
 constexpr int kFas = kFbm + 1;  // LDS row pitch of As
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -67,18 +69,20 @@ __device__ inline void stage_taps(float* __restrict__ Ht, const float* __restric
 
 // acc[t] += A[rows][c0 .. c0 + kFkc) x H[c0 .. c0 + kFkc)[cols of tile t]
 __device__ inline void mfma_chunk(const float* __restrict__ As, const float* __restrict__ Ht, int c0, int b0, int nu,
-                                  floatx16 (&acc)[2]) {
+                                  floatx16 (&acc)[kFnt]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int r = (w & 1) * 32 + (lane & 31), kl = lane >> 5;
-  const int col = b0 + (w >> 1) * 64 + (lane & 31);
+  const int col = b0 + (w >> 1) * (32 * kFnt) + (lane & 31);
   // H index of (c0 + kk + kl, col + 32 t) is c - b + nu - 1 + kFbn
   const int h0 = c0 + kl - col + nu - 1 + kFbn;
 #pragma unroll
   for (int kk = 0; kk < kFkc; kk += 2) {
     const float a = As[(kk + kl) * kFas + r];
-    const float bv0 = Ht[h0 + kk], bv1 = Ht[h0 + kk - 32];
-    acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv0, acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv1, acc[1], 0, 0, 0);
+    float bv[kFnt];
+#pragma unroll
+    for (int t = 0; t < kFnt; ++t) bv[t] = Ht[h0 + kk - 32 * t];
+#pragma unroll
+    for (int t = 0; t < kFnt; ++t) acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv[t], acc[t], 0, 0, 0);
   }
 }
 
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(256) void filter_fwd_mfma_kernel(const float* __res
   load(0);
   stage(A0, 0);
   __syncthreads();
-  floatx16 acc[2] = {};
+  floatx16 acc[kFnt] = {};
   // chunk k is multiplied from one buffer while chunk k + 1 is loaded and
   // staged into the other; one barrier per chunk
   int buf = 0;
@@ -132,8 +136,8 @@ __global__ __launch_bounds__(256) void filter_fwd_mfma_kernel(const float* __res
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float* dst = out + (size_t)view * fp.rows * fp.pitch;
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int b = b0 + (w >> 1) * 64 + 32 * t + (lane & 31);
+  for (int t = 0; t < kFnt; ++t) {
+    const int b = b0 + (w >> 1) * (32 * kFnt) + 32 * t + (lane & 31);
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int vo = v0 + (w & 1) * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i & 3);
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void filter_T_mfma_kernel(const float* __restr
   load(0);
   stage(A0, 0);
   __syncthreads();
-  floatx16 acc[2] = {};
+  floatx16 acc[kFnt] = {};
   int buf = 0;
   for (int c0 = 0; c0 < nu; c0 += kFkc) {
     const bool more = c0 + kFkc < nu;
@@ -188,32 +192,37 @@ __global__ __launch_bounds__(256) void filter_T_mfma_kernel(const float* __restr
     buf ^= 1;
   }
   // stage D as Ds[b_local][v_local] over the same LDS (the loop ended on a
-  // barrier), then write proj[b][v] along v
-  float* Ds = sm;  // [kFbn][kFas]
+  // barrier), two tiles per wave at a time, then write proj[b][v] along v.
+  // In pass h, Ds row bl holds b = b0 + 32 kFnt (bl / 64) + 64 h + bl % 64.
+  float* Ds = sm;  // [128][kFas]
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int bl = (w >> 1) * 64 + 32 * t + (lane & 31);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int vl = (w & 1) * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i & 3);
-      Ds[bl * kFas + vl] = acc[t][i];
-    }
-  }
-  __syncthreads();
   float* dst = proj + (size_t)view * nu * nv;
-  for (int i = threadIdx.x; i < kFbn * kFbm; i += 256) {
-    const int bl = i / kFbm, vl = i - bl * kFbm;
-    const int b = b0 + bl, v = v0 + vl;
-    if (b < nu && v < nv) dst[(size_t)b * nv + v] = fp.cs * Ds[bl * kFas + vl] * cos_weight(fp, b, v);
+#pragma unroll
+  for (int h = 0; h < kFnt / 2; ++h) {
+    if (h > 0) __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int bl = (w >> 1) * 64 + 32 * t + (lane & 31);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int vl = (w & 1) * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i & 3);
+        Ds[bl * kFas + vl] = acc[2 * h + t][i];
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 128 * kFbm; i += 256) {
+      const int bl = i / kFbm, vl = i - bl * kFbm;
+      const int b = b0 + (bl >> 6) * (32 * kFnt) + 64 * h + (bl & 63), v = v0 + vl;
+      if (b < nu && v < nv) dst[(size_t)b * nv + v] = fp.cs * Ds[bl * kFas + vl] * cos_weight(fp, b, v);
+    }
   }
 }
 
-// LDS bytes: two As buffers and the taps; the transposed filter also stages its 128 x 64
-// result over the same memory.
+// LDS bytes: two As buffers and the taps; the transposed filter also stages its
+// result, 128 x 64 at a time, over the same memory.
 static size_t filter_mfma_lds(int nu, bool transposed) {
   const size_t stage = 2 * (size_t)kFkc * kFas + 2 * (size_t)nu + kFkc + 2 * kFbn;
-  const size_t out = transposed ? (size_t)kFbn * kFas : 0;
+  const size_t out = transposed ? (size_t)128 * kFas : 0;
   return sizeof(float) * (stage > out ? stage : out);
 }
 
