@@ -207,6 +207,12 @@ int monte_primary_exact(monte_ctx *ctx, const monte_geom *g, const monte_phantom
  * ln(N0) - ln(clamp(N, 1, N0)) with N0 the photons aimed at the pixel (see photons_per_view). */
 int monte_counts_to_lineint(monte_ctx *ctx, const monte_geom *g, const double *counts,
                             uint64_t photons_per_view, float *lineint);
+/* The same against a measured flat field (air scan): ln(F) - ln(clamp(N, 1, F)),
+ * 0 where F <= 0.  flat is f64 [flat_views][nu][nv] with flat_views 1 (one air
+ * scan for every view) or n_views.  The reference has no flat field: it divides
+ * by the nominal photons per pixel (monte_cu/CBCT_real325.cu:264-294). */
+int monte_counts_to_lineint_flat(monte_ctx *ctx, const monte_geom *g, const double *counts,
+                                 const double *flat, int flat_views, float *lineint);
 
 /* ------------------------------------------------ phantom generation */
 /* Voxelise primitives into a label map (monte_cpp/make_image01.cpp:8-33 is the

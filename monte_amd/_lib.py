@@ -117,6 +117,7 @@ SIGNATURES = {
     "monte_mc_batch": (C.c_int, [_CTX, C.POINTER(McJob), C.c_int, C.POINTER(McStats)]),
     "monte_primary_exact": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.c_double, _P, _P]),
     "monte_counts_to_lineint": (C.c_int, [_CTX, C.POINTER(Geom), _P, C.c_uint64, _P]),
+    "monte_counts_to_lineint_flat": (C.c_int, [_CTX, C.POINTER(Geom), _P, _P, C.c_int, _P]),
     "monte_phantom_paint": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.POINTER(Primitive), C.c_int]),
     "monte_autotune": (C.c_int, [_CTX, C.c_int, C.POINTER(Geom), C.POINTER(Phantom), C.POINTER(McCfg),

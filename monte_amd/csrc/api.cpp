@@ -884,6 +884,25 @@ int monte_counts_to_lineint(monte_ctx* c, const monte_geom* g, const double* cou
   return st.back(lineint, dout, np * sizeof(float));
 }
 
+int monte_counts_to_lineint_flat(monte_ctx* c, const monte_geom* g, const double* counts, const double* flat,
+                                 int flat_views, float* lineint) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, false);
+  if (rc) return rc;
+  if (!counts || !flat || !lineint) return fail(c, MONTE_ERR_INVALID, "null buffer");
+  if (flat_views != 1 && flat_views != g->n_views)
+    return fail(c, MONTE_ERR_INVALID, "flat_views must be 1 or the number of views");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const size_t np = proj_elems(*g), nf = (size_t)flat_views * g->nu * g->nv;
+  Stage st{c};
+  const double* din = (const double*)st.in(kSlotStageA, counts, np * sizeof(double));
+  const double* dflat = (const double*)st.in(kSlotTuneA, flat, nf * sizeof(double));
+  float* dout = (float*)st.out(kSlotStageB, lineint, np * sizeof(float), false);
+  if (st.code) return st.code;
+  MONTE_HIP(c, launch_counts_to_lineint_flat(din, dflat, flat_views, dout, g->n_views, g->nu, g->nv, c->stream));
+  return st.back(lineint, dout, np * sizeof(float));
+}
+
 // ----------------------------------------------------------------- phantom
 int monte_phantom_paint(uint8_t* labels, int nx, int ny, int nz, double vx, double vy, double vz,
                         const monte_primitive* prims, int n_prims) {

@@ -231,5 +231,7 @@ hipError_t launch_primary_exact(const PrimaryGeom& P, const double* cs64, const 
                                 hipStream_t s);
 hipError_t launch_counts_to_lineint(const double* counts, float* out, int n_views, int nu, int nv,
                                     uint64_t photons_per_view, hipStream_t s);
+hipError_t launch_counts_to_lineint_flat(const double* counts, const double* flat, int flat_views, float* out,
+                                         int n_views, int nu, int nv, hipStream_t s);
 
 }  // namespace monte

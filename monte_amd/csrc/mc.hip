@@ -752,6 +752,30 @@ __global__ void counts_to_lineint_kernel(const double* __restrict__ counts, floa
   }
 }
 
+// Against a measured flat field (air scan) flat [flat_views][nu][nv], flat_views 1 or
+// n_views: ln(F) - ln(clamp(N, 1, F)), 0 where F <= 0.
+__global__ void counts_to_lineint_flat_kernel(const double* __restrict__ counts, const double* __restrict__ flat,
+                                              float* __restrict__ out, size_t n, size_t flat_n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double f = flat[i % flat_n];
+    double c = counts[i];
+    c = c > f ? f : c;
+    c = c < 1.0 ? 1.0 : c;
+    out[i] = f > 0.0 ? (float)(log(f) - log(c)) : 0.f;
+  }
+}
+
+hipError_t launch_counts_to_lineint_flat(const double* counts, const double* flat, int flat_views, float* out,
+                                         int n_views, int nu, int nv, hipStream_t s) {
+  const size_t npix = (size_t)nu * nv, n = (size_t)n_views * npix;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(counts_to_lineint_flat_kernel, dim3((unsigned)blocks), dim3(256), 0, s, counts, flat, out, n,
+                     (size_t)flat_views * npix);
+  return hipGetLastError();
+}
+
 hipError_t launch_counts_to_lineint(const double* counts, float* out, int n_views, int nu, int nv,
                                     uint64_t photons_per_view, hipStream_t s) {
   const size_t n = (size_t)n_views * nu * nv;

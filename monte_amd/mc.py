@@ -157,6 +157,26 @@ def counts_to_lineint(counts: torch.Tensor, geom: Geometry, photons_per_view: in
     return out
 
 
+def counts_to_lineint_flat(counts: torch.Tensor, geom: Geometry, flat: torch.Tensor, ctx: Context | None = None):
+    """ln(F) - ln(clamp(N, 1, F)) against a measured flat field F: float64 [nu][nv]
+    (one air scan for every view) or [views][nu][nv]; 0 where F <= 0."""
+    ctx = ctx or default_context(counts.device.index or 0)
+    if counts.dtype != torch.float64 or tuple(counts.shape) != geom.proj_shape:
+        raise TypeError(f"counts must be float64 {geom.proj_shape}")
+    pix = (geom.nu, geom.nv)
+    if flat.dtype != torch.float64 or tuple(flat.shape) not in (pix, (1, *pix), geom.proj_shape):
+        raise TypeError(f"flat must be float64 {pix} or {geom.proj_shape}")
+    if not counts.is_cuda or flat.device != counts.device:
+        raise TypeError("counts and flat must be on the same CUDA device")
+    counts, flat = counts.contiguous(), flat.contiguous()
+    flat_views = 1 if flat.numel() == geom.nu * geom.nv else geom.n_views
+    out = torch.empty(geom.proj_shape, dtype=torch.float32, device=counts.device)
+    g = geom.c()
+    ctx.call("monte_counts_to_lineint_flat", C.byref(g), C.c_void_p(counts.data_ptr()), C.c_void_p(flat.data_ptr()),
+             flat_views, C.c_void_p(out.data_ptr()))
+    return out
+
+
 def autotune(ph: LabelPhantom | None, geom: Geometry, settings: McSettings | None, what: int = 3,
              labels_dev: torch.Tensor | None = None, ctx: Context | None = None, device: int = 0) -> dict:
     """Times candidate launch shapes and keeps the fastest in the context."""

@@ -125,3 +125,41 @@ def test_counts_to_lineint(ctx):
     assert np.array_equal(n0[0], photons_per_pixel(geom, N))
     ref = np.log(n0) - np.log(np.clip(counts, 1.0, n0))
     assert np.allclose(got, ref, rtol=1e-6, atol=1e-6)
+
+
+def test_counts_to_lineint_flat(ctx):
+    import ctypes as C
+
+    import torch
+
+    from monte_amd import _lib
+    from monte_amd import geometry as G
+    from monte_amd.mc import counts_to_lineint, counts_to_lineint_flat, photons_per_pixel
+
+    geom = G.small(n_views=3)
+    N = 61 * geom.nu * geom.nv + 1000
+    rng = np.random.default_rng(8)
+    counts = rng.integers(0, 70, size=geom.proj_shape).astype(np.float64)
+    cd = torch.from_numpy(counts).cuda()
+    # the nominal photons per pixel as the flat field gives the plain conversion, bitwise
+    n0 = photons_per_pixel(geom, N)
+    same = counts_to_lineint_flat(cd, geom, torch.from_numpy(n0).cuda(), ctx)
+    assert torch.equal(same, counts_to_lineint(cd, geom, N, ctx))
+    # one air scan for every view, and one per view, with dead pixels (F <= 0 -> 0)
+    for flat in (rng.uniform(40.0, 80.0, size=(geom.nu, geom.nv)), rng.uniform(40.0, 80.0, size=geom.proj_shape)):
+        flat[..., :3, :2] = 0.0
+        flat[..., 5, 7] = -1.0
+        got = counts_to_lineint_flat(cd, geom, torch.from_numpy(flat).cuda(), ctx).cpu().numpy()
+        f = np.broadcast_to(flat, geom.proj_shape)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ref = np.where(f > 0, np.log(f) - np.log(np.maximum(np.minimum(counts, f), 1.0)), 0.0)
+        assert np.allclose(got, ref.astype(np.float32), rtol=1e-6, atol=1e-6)
+        assert np.all(got[..., :3, :2] == 0.0) and np.all(got[..., 5, 7] == 0.0)
+    # flat_views must be 1 or the number of views
+    L = _lib.lib()
+    out = torch.empty(geom.proj_shape, dtype=torch.float32, device="cuda")
+    fl = torch.ones((2, geom.nu, geom.nv), dtype=torch.float64, device="cuda")
+    g = geom.c()
+    rc = L.monte_counts_to_lineint_flat(ctx.handle, C.byref(g), C.c_void_p(cd.data_ptr()), C.c_void_p(fl.data_ptr()),
+                                        2, C.c_void_p(out.data_ptr()))
+    assert rc == _lib.ERR_INVALID
