@@ -215,3 +215,46 @@ def test_primary_exact_box_closed_form(ctx):
     err = np.abs(got - ref) / np.maximum(ref, 1e-3 * ref.max())
     print("box closed form: max rel", err.max(), "max L", ref.max())
     assert err.max() < 1e-5
+
+
+def test_woodcock_majorant_invariance(ctx):
+    """A label that no voxel carries, made dense, raises the Woodcock majorant (more
+    virtual steps per history) without changing the physics: both runs sample the
+    same distributions, so their tallies agree pixel by pixel within Poisson noise."""
+    import torch
+
+    from monte_amd import geometry as G
+    from monte_amd import phantoms, tables
+    from monte_amd.mc import McSettings, mc_project
+
+    geom = G.small(n_views=4)
+    base = phantoms.small_phantom()
+    heavy = phantoms.LabelPhantom(base.labels, base.voxel, list(base.materials) + [(tables.CALCIUM, 5.0)],
+                                  base.origin)
+    assert 3 not in np.unique(base.labels)
+    s = McSettings(photons_per_view=4_000_000, e_kev=60.0)
+    pa, sa, sta = mc_project(base, geom, s, ctx=ctx)
+    pb, sb, stb = mc_project(heavy, geom, s, ctx=ctx)
+    print("steps per history", sta["steps"] / sta["histories"], "->", stb["steps"] / stb["histories"])
+    assert stb["steps"] > 1.5 * sta["steps"]
+    print("tallied scatter", sta["tallied_scatter"], stb["tallied_scatter"])
+
+    def bins(t, k):  # k x k pixel blocks, so the sparse scatter image has counts to compare
+        x = t.cpu().numpy()
+        return x.reshape(x.shape[0], x.shape[1] // k, k, x.shape[2] // k, k).sum(axis=(2, 4))
+
+    for name, a, b in (("primary", bins(pa, 8), bins(pb, 8)), ("scatter", bins(sa, 16), bins(sb, 16))):
+        keep = a + b >= 20
+        chi2 = float(np.mean((a - b)[keep] ** 2 / (a + b)[keep]))
+        dof = int(keep.sum())
+        tot = (a.sum() - b.sum()) / np.sqrt(a.sum() + b.sum())
+        print(f"{name}: bins {dof}  chi2/dof {chi2:.4f}  total difference {tot:.2f} sigma")
+        assert dof > 100
+        tol = 5 * np.sqrt(2.0 / dof)
+        if name == "scatter":  # one count per history at most: Poisson bins, chi2/dof near 1
+            assert abs(chi2 - 1.0) < tol
+        else:  # binomial bins: a + b over-estimates their variance, so only an excess counts
+            assert chi2 < 1.0 + tol
+        assert abs(tot) < 5.0
+    for k in ("tallied_primary", "tallied_scatter", "photo", "compton", "coherent"):
+        assert abs(sta[k] - stb[k]) < 6 * np.sqrt(max(sta[k] + stb[k], 1)), k
