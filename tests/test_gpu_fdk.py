@@ -117,3 +117,35 @@ def test_batch_equals_single(ctx):
     vb = fdk(proj, geom, ctx)
     for b in range(3):
         assert torch.equal(vb[b], fdk(proj[b].contiguous(), geom, ctx))
+
+
+def _rot_equivariance(ctx, geom, seed):
+    """Views rolled by a quarter turn reconstruct the volume turned by 90 degrees
+    about z (in the [y][x] plane), and the adjoint commutes the other way."""
+    import torch
+
+    from monte_amd.fdk import fdk, fdk_adjoint
+
+    q = geom.n_views // 4
+    assert q * 4 == geom.n_views and geom.nx == geom.ny
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    P = torch.randn(geom.proj_shape, generator=g, device="cuda")
+    V = fdk(P, geom, ctx)
+    V2 = fdk(torch.roll(P, -q, dims=0), geom, ctx)
+    R = torch.rot90(V, 1, dims=(1, 2))
+    e_fwd = float((V2 - R).abs().max() / R.abs().max())
+    Y = torch.randn(geom.vol_shape, generator=g, device="cuda")
+    A = fdk_adjoint(Y, geom, ctx)
+    A2 = fdk_adjoint(torch.rot90(Y, 1, dims=(1, 2)).contiguous(), geom, ctx)
+    Ar = torch.roll(A, -q, dims=0)
+    e_adj = float((A2 - Ar).abs().max() / Ar.abs().max())
+    print(f"quarter-turn equivariance: fdk {e_fwd:.2e}  adjoint {e_adj:.2e}", flush=True)
+    return e_fwd, e_adj
+
+
+def test_quarter_turn_equivariance(ctx):
+    from monte_amd import geometry as G
+
+    geom = G.small(n_views=36).replace(u_off=0.05, v_off=0.07)
+    e_fwd, e_adj = _rot_equivariance(ctx, geom, 3)
+    assert e_fwd < 1e-5 and e_adj < 1e-5

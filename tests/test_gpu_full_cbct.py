@@ -133,3 +133,11 @@ def test_full_mc_primary_unbiased(ctx, full):
     assert abs(bias) < 5.0
     # photons through air only are never absorbed
     assert np.all(a[L == 0] == n0)
+
+
+def test_full_quarter_turn_equivariance(ctx, full):
+    from test_gpu_fdk import _rot_equivariance
+
+    geom, _, _ = full
+    e_fwd, e_adj = _rot_equivariance(ctx, geom, 5)
+    assert e_fwd < 1e-5 and e_adj < 1e-5
