@@ -105,6 +105,27 @@ def test_standard_stages_match_oracle(ctx, oracle):
     assert rel_max(v, v_ref) < TOL
 
 
+@pytest.mark.parametrize("nu,nv", [(300, 77), (257, 33), (33, 257), (1000, 40)])
+def test_filter_ragged_detectors(ctx, oracle, nu, nv):
+    """Detector sizes that leave partial product tiles (64 rows of v, 256 columns
+    of u, 32-wide chunks of the sum) in both filter directions."""
+    from monte_amd import geometry as G
+    from monte_amd.fdk import fdk_filter, fdk_filter_adjoint
+
+    geom = G.small(n_views=3).replace(nu=nu, nv=nv, du=12.8 / nu, dv=12.8 / nv, u_off=0.03)
+    rng = np.random.default_rng(nu + nv)
+    og = oracle_geom(oracle, geom)
+    proj = rng.standard_normal(geom.proj_shape).astype(np.float32)
+    f_ref = oracle.fdk_std_filter(og, oracle.fdk_std_weight(og, proj))
+    f = fdk_filter(_t(proj), geom, ctx).cpu().numpy()
+    filt = rng.standard_normal(geom.filt_shape).astype(np.float32)
+    t_ref = oracle.fdk_std_weight(og, oracle.fdk_std_filter_T(og, filt))  # the weight is diagonal
+    t = fdk_filter_adjoint(_t(filt), geom, ctx).cpu().numpy()
+    print(f"nu {nu} nv {nv}: filter {rel_max(f, f_ref):.2e}  transposed {rel_max(t, t_ref):.2e}")
+    assert rel_max(f, f_ref) < TOL
+    assert rel_max(t, t_ref) < TOL
+
+
 def test_batch_equals_single(ctx):
     import torch
 
