@@ -313,8 +313,10 @@ __device__ inline int label_of(uint32_t i, const McJob& J) {
 }
 
 // A parked branch runs once at least this many lanes wait for it (or nothing
-// else can run).
-constexpr unsigned kInteractBatch = 16;
+// else can run).  Interactions wait for 24 lanes: each batch costs a whole
+// Klein-Nishina rejection loop and a flight set-up, so fuller batches pay
+// (20 to 28 measured equal, 1.3% faster than 16 on the full workload).
+constexpr unsigned kInteractBatch = 24;
 constexpr unsigned kRefillBatch = 16;
 
 __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restrict__ jobs, int n_jobs,
