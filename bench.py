@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--views", type=int, default=360)
     ap.add_argument("--batch", type=int, default=8, help="stacks in the differentiable-FDK step")
     ap.add_argument("--cpu-seconds", type=float, default=24.0, help="CPU-baseline budget (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU-baseline threads (0 = every CPU this process may use, capped by OMP_NUM_THREADS)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: time candidate launch shapes first (the defaults are this workload's best)")
     return ap.parse_args()
@@ -192,6 +193,32 @@ def profiled_traffic(kernel, a, photons):
     return rec["bytes_per_launch"] if rec else None
 
 
+def host_cpus() -> dict:
+    """The host CPUs this run may use: every CPU of this process's affinity
+    set, capped by OMP_NUM_THREADS where the environment sets it (a GPU box
+    gives each GPU a share of its host CPUs and says so there), with the
+    machine's own count and CPU model for the record."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    share = affinity
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        share = min(share, int(env))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"share": share, "affinity": affinity, "machine": os.cpu_count(), "omp_num_threads": env or None,
+            "model": model}
+
+
 def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
     """Oracle (FP64 C restatement) on the host cores, on a bounded sample of
     about half the budget each for MC and FDK: a probe run, then up to two runs
@@ -203,7 +230,8 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
 
     from monte_amd import tables
 
-    nthreads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    host = host_cpus()
+    nthreads = a.cpu_threads or host["share"]
     og = O.geom(**geom.as_dict())
     oph = O.phantom(ph.labels, *ph.voxel, materials=ph.materials)
     tb = O.tables(tables.reference_tables())
@@ -224,7 +252,7 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
             break
         n = max(n, int(n * target / max(dt, 1e-3)))
         dt = mc_time(n)
-    mc = {"value": 2 * n / dt, "unit": "histories/s", "cores": nthreads, "kind": "port",
+    mc = {"value": 2 * n / dt, "unit": "histories/s", "cores": nthreads, "kind": "port", "host": host,
           "sample": f"oracle MC (FP64, same physics and Philox streams) on 2 views x {n} photons of the "
                     f"workload, {dt:.1f} s", "gpu_over_cpu": gpu_hps / (2 * n / dt)}
 
@@ -243,7 +271,7 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
         nv = int(min(geom.n_views, max(nv + 1, nv * target / max(dt, 1e-3))))
         dt = fdk_time(nv)
     ups = nv * geom.nx * geom.ny * geom.nz / dt
-    fdk = {"value": ups / 1e9, "unit": "GUPS", "cores": nthreads, "kind": "port",
+    fdk = {"value": ups / 1e9, "unit": "GUPS", "cores": nthreads, "kind": "port", "host": host,
            "sample": f"oracle FDK (FP64: weight, direct-convolution filter, backprojection) of {nv} views "
                      f"onto the full {geom.nx}^3 volume, {dt:.1f} s", "gpu_over_cpu": gpu_ups / ups}
     return mc, fdk

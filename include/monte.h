@@ -96,7 +96,9 @@ typedef struct monte_geom {
 } monte_geom;
 
 /* Voxel label phantom ("phantom in").  Label l has attenuation table
- * mat_table[l] (an index into the loaded tables, -1 = vacuum) at density[l]. */
+ * mat_table[l] (an index into the loaded tables, -1 = vacuum) at density[l].
+ * Labels at or above n_labels are vacuum; labels above 15 read as label 15
+ * (transport, exact primary and flight box alike). */
 typedef struct monte_phantom {
   const uint8_t *labels;    /* [nz][ny][nx] */
   int32_t nx, ny, nz;
@@ -218,12 +220,19 @@ int monte_counts_to_lineint_flat(monte_ctx *ctx, const monte_geom *g, const doub
 /* Voxelise primitives into a label map (monte_cpp/make_image01.cpp:8-33 is the
  * reference's single-sphere generator).  Each primitive paints its label over
  * voxel centres inside it, in order. */
-typedef enum monte_shape { MONTE_SHAPE_ELLIPSOID = 0, MONTE_SHAPE_CYLINDER_Z = 1, MONTE_SHAPE_BOX = 2 } monte_shape;
+typedef enum monte_shape {
+  MONTE_SHAPE_ELLIPSOID = 0,
+  MONTE_SHAPE_CYLINDER_Z = 1,
+  MONTE_SHAPE_BOX = 2,
+  MONTE_SHAPE_CYLINDER_X = 3  /* axis along x, as the reference's water cylinder and Ca rods
+                                 (monte_cu/CBCT_real325.cu:916-928) */
+} monte_shape;
 typedef struct monte_primitive {
   int32_t shape;            /* monte_shape */
   int32_t label;
   double cx, cy, cz;        /* centre (cm) */
-  double rx, ry, rz;        /* semi-axes (cylinder: rx, ry radii, rz half-height) */
+  double rx, ry, rz;        /* semi-axes (cylinder along z: rx, ry radii, rz half-height;
+                               along x: ry, rz radii, rx half-length) */
 } monte_primitive;
 int monte_phantom_paint(uint8_t *labels, int nx, int ny, int nz, double vx, double vy, double vz,
                         const monte_primitive *prims, int n_prims); /* host labels */

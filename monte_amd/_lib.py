@@ -18,7 +18,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_IO, ERR_UNSUPPORTED = 0, -1,
 PRESET_STANDARD, PRESET_BP3D20 = 0, 1
 FILTER_RAMLAK, FILTER_SHEPP_LOGAN = 0, 1
 COH_FORWARD, COH_RAYLEIGH = 0, 1
-SHAPE_ELLIPSOID, SHAPE_CYLINDER_Z, SHAPE_BOX = 0, 1, 2
+SHAPE_ELLIPSOID, SHAPE_CYLINDER_Z, SHAPE_BOX, SHAPE_CYLINDER_X = 0, 1, 2, 3
 
 
 class MonteError(RuntimeError):

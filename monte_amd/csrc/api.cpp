@@ -272,12 +272,16 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
     return fail(c, MONTE_ERR_INVALID, "bad phantom dimensions");
   if ((double)ph->nx * ph->ny * ph->nz > 4294967295.0)
     return fail(c, MONTE_ERR_INVALID, "phantom has more than 2^32 - 1 voxels");
-  if ((double)ph->ny * ph->nz > 16777216.0 || ph->nx > 16777216)
-    return fail(c, MONTE_ERR_UNSUPPORTED, "phantom ny * nz and nx must not exceed 2^24 (24-bit voxel indexing)");
+  // the voxel index is two 24-bit multiply-adds: every operand, ny and the
+  // partial index iz * ny + iy < ny * nz included, must fit in 24 bits
+  if ((double)ph->ny * ph->nz >= 16777216.0 || ph->nx >= 16777216)
+    return fail(c, MONTE_ERR_UNSUPPORTED, "phantom ny * nz and nx must be below 2^24 (24-bit voxel indexing)");
   if (ph->n_labels < 1 || ph->n_labels > kMaxLabels) return fail(c, MONTE_ERR_INVALID, "n_labels out of range");
   if (c->n_tables < 1) return fail(c, MONTE_ERR_STATE, "attenuation tables not loaded (monte_load_tables)");
-  if (cfg->photons_per_view == 0 || cfg->photons_per_view > 0xFFFFFFFFull)
-    return fail(c, MONTE_ERR_INVALID, "photons_per_view must be in [1, 2^32)");
+  // photon indices are 32-bit: a unit's end (next + batch) and the refill
+  // cursor (next + 64) must not wrap
+  if (cfg->photons_per_view == 0 || cfg->photons_per_view > 0x100000000ull - (uint64_t)batch - 64)
+    return fail(c, MONTE_ERR_INVALID, "photons_per_view must be in [1, 2^32 - batch - 64] (batch = 64 * mc_batch)");
   if (cfg->max_interactions < 0) return fail(c, MONTE_ERR_INVALID, "max_interactions < 0");
   if (!cfg->spectrum && !(cfg->e_kev >= 1.0 && cfg->e_kev <= kRows - 1))
     return fail(c, MONTE_ERR_INVALID, "e_kev must be in [1, 200]");
@@ -371,22 +375,44 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
   return MONTE_OK;
 }
 
-// Lay out jobs, tables, CDFs and view tables in the MC slot and launch.
-int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blocks_per_cu, float* ms_out,
-           std::vector<McCounters>* counters_out) {
+// Where a launch's job descriptors, tables, CDFs, view tables, queue, counters
+// and flight boxes sit in its device buffer.
+struct McLayout {
+  size_t jobs_off = 0, tab_off = 0, q_off = 0, cnt_off = 0, box_off = 0, total = 0;
+  std::vector<size_t> cdf_off, cs_off;
+};
+
+McLayout mc_layout(const std::vector<HostJob>& hj) {
   const int n = (int)hj.size();
+  McLayout L;
   size_t off = 0;
-  const size_t jobs_off = off; off = align_up(off + sizeof(McJob) * n, 256);
-  const size_t tab_off = off; off = align_up(off + sizeof(McTables) * n, 256);
-  std::vector<size_t> cdf_off(n), cs_off(n);
+  L.jobs_off = off; off = align_up(off + sizeof(McJob) * n, 256);
+  L.tab_off = off; off = align_up(off + sizeof(McTables) * n, 256);
+  L.cdf_off.resize(n);
+  L.cs_off.resize(n);
   for (int i = 0; i < n; ++i) {
-    cdf_off[i] = off; off = align_up(off + sizeof(float) * hj[i].cdf.size(), 256);
-    cs_off[i] = off; off = align_up(off + sizeof(float2) * hj[i].view_cs.size(), 256);
+    L.cdf_off[i] = off; off = align_up(off + sizeof(float) * hj[i].cdf.size(), 256);
+    L.cs_off[i] = off; off = align_up(off + sizeof(float2) * hj[i].view_cs.size(), 256);
   }
-  const size_t q_off = off; off = align_up(off + 256, 256);
-  const size_t cnt_off = off; off = align_up(off + sizeof(McCounters) * n, 256);
-  const size_t box_off = off; off = align_up(off + sizeof(int) * 8 * n, 256);
-  char* base = (char*)workspace(c, kSlotMc, off);
+  L.q_off = off; off = align_up(off + 256, 256);
+  L.cnt_off = off; off = align_up(off + sizeof(McCounters) * n, 256);
+  L.box_off = off; off = align_up(off + sizeof(int) * 8 * n, 256);
+  L.total = off;
+  return L;
+}
+
+// Lay out jobs, tables, CDFs and view tables in a device buffer and launch.
+// The buffer is the context's MC workspace, or `base` (at least
+// mc_layout(hj).total bytes) when the caller provides one.
+int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blocks_per_cu, float* ms_out,
+           std::vector<McCounters>* counters_out, char* base = nullptr) {
+  const int n = (int)hj.size();
+  const McLayout Lo = mc_layout(hj);
+  const size_t jobs_off = Lo.jobs_off, tab_off = Lo.tab_off, q_off = Lo.q_off, cnt_off = Lo.cnt_off,
+               box_off = Lo.box_off;
+  const std::vector<size_t>& cdf_off = Lo.cdf_off;
+  const std::vector<size_t>& cs_off = Lo.cs_off;
+  if (!base) base = (char*)workspace(c, kSlotMc, Lo.total);
   if (!base) return fail(c, MONTE_ERR_NOMEM, "MC descriptor allocation failed");
 
   // Flights are clipped to the bounding box of the voxels that attenuate at
@@ -434,7 +460,10 @@ int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blo
     units += J.unit_count;
     jobs[i] = J;
   }
-  if (units > 0xFFFFFFFFull) return fail(c, MONTE_ERR_INVALID, "too many work units in one launch (raise the batch)");
+  // the 32-bit queue counter passes the last unit by one fetch per wave
+  const int grid_waves = std::max(1, c->n_cu * blocks_per_cu) * (block / 64);
+  if (units > 0xFFFFFFFFull - (uint64_t)grid_waves)
+    return fail(c, MONTE_ERR_INVALID, "too many work units in one launch (raise the batch)");
   MONTE_HIP(c, hipMemcpyAsync(base + jobs_off, jobs.data(), sizeof(McJob) * n, hipMemcpyHostToDevice, c->stream));
   for (int i = 0; i < n; ++i) {
     MONTE_HIP(c, hipMemcpyAsync(base + tab_off + sizeof(McTables) * i, &hj[i].tables, sizeof(McTables),
@@ -448,21 +477,16 @@ int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blo
   unsigned int* queue = (unsigned int*)(base + q_off);
   McCounters* counters = (McCounters*)(base + cnt_off);
   MONTE_HIP(c, launch_mc_reset(queue, counters, n, c->stream));
-  hipEvent_t e0, e1;
-  MONTE_HIP(c, hipEventCreate(&e0));
-  MONTE_HIP(c, hipEventCreate(&e1));
-  MONTE_HIP(c, hipEventRecord(e0, c->stream));
+  MONTE_HIP(c, hipEventRecord(c->ev0, c->stream));
   const int grid = std::max(1, c->n_cu * blocks_per_cu);
   hipError_t le = launch_mc_transport((const McJob*)(base + jobs_off), n, units, queue, counters, batch, block, grid,
                                       c->stream);
-  MONTE_HIP(c, hipEventRecord(e1, c->stream));
+  MONTE_HIP(c, hipEventRecord(c->ev1, c->stream));
   std::vector<McCounters> cnt(n);
   MONTE_HIP(c, hipMemcpyAsync(cnt.data(), counters, sizeof(McCounters) * n, hipMemcpyDeviceToHost, c->stream));
   MONTE_HIP(c, hipStreamSynchronize(c->stream));
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
   if (le != hipSuccess) return fail(c, MONTE_ERR_HIP, std::string("transport launch: ") + hipGetErrorString(le));
   if (ms_out) *ms_out = ms;
   if (counters_out) *counters_out = cnt;
@@ -524,6 +548,12 @@ int monte_create(monte_ctx** out, int device) {
     return MONTE_ERR_HIP;
   }
   c->stream = c->own_stream;
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return MONTE_ERR_HIP;
+  }
   int cu = 0;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->n_cu = cu;
   c->tuning = default_tuning();
@@ -539,6 +569,8 @@ void monte_destroy(monte_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (int i = 0; i < monte_ctx::kSlots; ++i)
     if (c->ws[i]) (void)hipFree(c->ws[i]);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
@@ -910,7 +942,7 @@ int monte_phantom_paint(uint8_t* labels, int nx, int ny, int nz, double vx, doub
       (n_prims > 0 && !prims))
     return MONTE_ERR_INVALID;
   for (int i = 0; i < n_prims; ++i)
-    if (prims[i].label < 0 || prims[i].label > 255 || prims[i].shape < 0 || prims[i].shape > 2 ||
+    if (prims[i].label < 0 || prims[i].label > 255 || prims[i].shape < 0 || prims[i].shape > 3 ||
         !(prims[i].rx > 0) || !(prims[i].ry > 0) || !(prims[i].rz > 0))
       return MONTE_ERR_INVALID;
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -928,6 +960,7 @@ int monte_phantom_paint(uint8_t* labels, int nx, int ny, int nz, double vx, doub
             bool in;
             if (p.shape == MONTE_SHAPE_ELLIPSOID) in = a * a + b * b + cc * cc <= 1.0;
             else if (p.shape == MONTE_SHAPE_CYLINDER_Z) in = a * a + b * b <= 1.0 && std::fabs(cc) <= 1.0;
+            else if (p.shape == MONTE_SHAPE_CYLINDER_X) in = b * b + cc * cc <= 1.0 && std::fabs(a) <= 1.0;
             else in = std::fabs(a) <= 1.0 && std::fabs(b) <= 1.0 && std::fabs(cc) <= 1.0;
             if (in) row[x] = (uint8_t)p.label;
           }

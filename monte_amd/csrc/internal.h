@@ -131,6 +131,7 @@ struct monte_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // times transport launches (created once, with the context)
   bool host_ptrs = false;
   std::string err;
   int n_cu = 256;
@@ -179,6 +180,7 @@ int fail(monte_ctx* c, int code, const std::string& msg);
 
 // workspace slot i grown to at least bytes; returns nullptr on failure
 void* workspace(monte_ctx* c, int slot, size_t bytes);
+
 
 // ---------------------------------------------------------- kernel launches
 // FDK (fdk.hip)

@@ -105,14 +105,17 @@ __device__ inline float thomson_mu(float u) {
   return a < 0.f ? -m : m;
 }
 
-// Rotate unit direction d by polar cosine c and azimuth phi.
+// Rotate unit direction d by polar cosine c and azimuth phi: c d + s (cos(phi)
+// e1 + sin(phi) e2) with e1 = (dx dz, dy dz, -r^2)/r, e2 = (-dy, dx, 0)/r and
+// r = sqrt(dx^2 + dy^2), which stays accurate near the poles (1 - dz^2 would
+// cancel); only for d on the z axis is the pair the x and y axes.
 __device__ inline void rotate_dir(float& dx, float& dy, float& dz, float c, float phi) {
   const float s = sqrtf(fmaxf(0.f, 1.f - c * c));
   float sp, cp;
   sincosf(phi, &sp, &cp);
-  const float r = sqrtf(fmaxf(0.f, 1.f - dz * dz));
+  const float r = sqrtf(dx * dx + dy * dy);
   float nx, ny, nz;
-  if (r > 1e-5f) {
+  if (r > 1e-15f) {  // r^2 stays a normal FP32 number
     const float ir = 1.0f / r;
     nx = c * dx + s * (dx * dz * cp - dy * sp) * ir;
     ny = c * dy + s * (dy * dz * cp + dx * sp) * ir;
@@ -210,10 +213,12 @@ enum { kStatHistories, kStatSteps, kStatInteractions, kStatPhoto, kStatCoherent,
 
 __device__ inline unsigned popc(uint64_t m) { return (unsigned)__popcll(m); }
 
+// Sum over the wave in 64 bits: 64 lanes of 32-bit counts can exceed 2^32.
 __device__ inline unsigned long long wave_sum(uint32_t v) {
+  unsigned long long s = v;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
 }
 
 // Straight line from the current position to the detector plane x' = sdd - sod.
@@ -302,7 +307,7 @@ __device__ inline uint32_t voxel_at(const Photon& p, float t, const McJob& J, fl
   const int iy = min(max((int)fmaf(qy, J.inv_vy, J.iy0), 0), J.ny - 1);
   const int iz = min(max((int)fmaf(qz, J.inv_vz, J.iz0), 0), J.nz - 1);
   // 32-bit index from two 24-bit multiply-adds: the host checks
-  // nz * ny <= 2^24, nx <= 2^24 and nx * ny * nz < 2^32
+  // nz * ny < 2^24, nx < 2^24 and nx * ny * nz < 2^32
   return mad_u24(mad_u24((uint32_t)iz, (uint32_t)J.ny, (uint32_t)iy), (uint32_t)J.nx, (uint32_t)ix);
 }
 
@@ -685,7 +690,8 @@ __global__ __launch_bounds__(256) void primary_exact_kernel(PrimaryGeom P, const
       if (sn[2] < sn[a]) a = 2;
       const double te = fmin(sn[a], t1);
       if (te > t) {
-        const int lab = labels[((size_t)idx[2] * P.n[1] + idx[1]) * P.n[0] + idx[0]];
+        // labels past the table read as the last label, as in transport (label_of)
+        const int lab = min((int)labels[((size_t)idx[2] * P.n[1] + idx[1]) * P.n[0] + idx[0]], kMaxLabels - 1);
         lens[lab][tid] += te - t;
         t = te;
       }

@@ -64,6 +64,26 @@ def device_labels(ph: LabelPhantom, device) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(ph.labels)).to(device)
 
 
+def _check_labels(labels: torch.Tensor, ph: LabelPhantom, ctx: Context) -> None:
+    """The kernels read nx * ny * nz bytes of labels on the context's device."""
+    if not isinstance(labels, torch.Tensor) or labels.dtype != torch.uint8 or not labels.is_contiguous():
+        raise TypeError("labels must be a contiguous uint8 tensor")
+    if not labels.is_cuda or labels.device.index != ctx.device:
+        raise TypeError(f"labels must be on cuda:{ctx.device}")
+    if labels.numel() != ph.labels.size:
+        raise ValueError(f"labels hold {labels.numel()} voxels, the phantom {ph.labels.size}")
+
+
+def _check_tally(t: torch.Tensor, geom: Geometry, ctx: Context, what: str) -> None:
+    """Tallies take FP64 atomics over [views][nu][nv] on the context's device."""
+    shape = geom.proj_shape
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or tuple(t.shape) != shape \
+            or not t.is_contiguous():
+        raise TypeError(f"{what} must be a contiguous float64 tensor of shape {shape}")
+    if not t.is_cuda or t.device.index != ctx.device:
+        raise TypeError(f"{what} must be on cuda:{ctx.device}")
+
+
 def mc_project(ph: LabelPhantom, geom: Geometry, settings: McSettings, labels_dev: torch.Tensor | None = None,
                primary: torch.Tensor | None = None, scatter: torch.Tensor | None = None,
                ctx: Context | None = None, device: int = 0):
@@ -72,16 +92,14 @@ def mc_project(ph: LabelPhantom, geom: Geometry, settings: McSettings, labels_de
     dev = torch.device("cuda", ctx.device)
     if labels_dev is None:
         labels_dev = device_labels(ph, dev)
-    if labels_dev.dtype != torch.uint8 or not labels_dev.is_contiguous():
-        raise TypeError("labels must be a contiguous uint8 tensor")
+    _check_labels(labels_dev, ph, ctx)
     shape = geom.proj_shape
     if primary is None:
         primary = torch.zeros(shape, dtype=torch.float64, device=dev)
     if scatter is None:
         scatter = torch.zeros(shape, dtype=torch.float64, device=dev)
-    for t in (primary, scatter):
-        if t.dtype != torch.float64 or tuple(t.shape) != shape or not t.is_contiguous():
-            raise TypeError(f"tallies must be contiguous float64 {shape}")
+    _check_tally(primary, geom, ctx, "primary")
+    _check_tally(scatter, geom, ctx, "scatter")
     g = geom.c()
     p = ph.c(labels_dev.data_ptr())
     cfg, _keep = settings.c()
@@ -114,6 +132,9 @@ def mc_batch(jobs: list, ctx: Context | None = None, device: int = 0):
             j.primary = torch.zeros(j.geom.proj_shape, dtype=torch.float64, device=dev)
         if j.scatter is None:
             j.scatter = torch.zeros(j.geom.proj_shape, dtype=torch.float64, device=dev)
+        _check_labels(j.labels_dev, j.phantom, ctx)
+        _check_tally(j.primary, j.geom, ctx, f"job {i} primary")
+        _check_tally(j.scatter, j.geom, ctx, f"job {i} scatter")
         g = j.geom.c()
         p = j.phantom.c(j.labels_dev.data_ptr())
         cfg, k = j.settings.c()
@@ -134,6 +155,7 @@ def primary_exact(ph: LabelPhantom, geom: Geometry, e_kev: float = 60.0, spectru
     dev = torch.device("cuda", ctx.device)
     if labels_dev is None:
         labels_dev = device_labels(ph, dev)
+    _check_labels(labels_dev, ph, ctx)
     out = torch.empty(geom.proj_shape, dtype=torch.float32, device=dev)
     g = geom.c()
     p = ph.c(labels_dev.data_ptr())
@@ -149,6 +171,8 @@ def counts_to_lineint(counts: torch.Tensor, geom: Geometry, photons_per_view: in
     ctx = ctx or default_context(counts.device.index or 0)
     if counts.dtype != torch.float64 or tuple(counts.shape) != geom.proj_shape:
         raise TypeError(f"counts must be float64 {geom.proj_shape}")
+    if not counts.is_cuda or counts.device.index != ctx.device:
+        raise TypeError(f"counts must be on cuda:{ctx.device}")
     counts = counts.contiguous()
     out = torch.empty(geom.proj_shape, dtype=torch.float32, device=counts.device)
     g = geom.c()
@@ -188,6 +212,7 @@ def autotune(ph: LabelPhantom | None, geom: Geometry, settings: McSettings | Non
     if what & 1:
         if labels_dev is None:
             labels_dev = device_labels(ph, dev)
+        _check_labels(labels_dev, ph, ctx)
         pp = ph.c(labels_dev.data_ptr())
         cc, keep = settings.c()
     t = _lib.Tuning()

@@ -99,3 +99,17 @@ def cube_phantom(side: int, v: float, seed: int) -> LabelPhantom:
     prims = [(_lib.SHAPE_ELLIPSOID, 1, (0.0, 0.0, 0.0), tuple(a)),
              (_lib.SHAPE_ELLIPSOID, 2, tuple(c), (r, r, r))]
     return LabelPhantom(paint((side, side, side), (v, v, v), prims), (v, v, v), list(MATERIALS))
+
+
+def reference_rods_phantom(v: float = 0.1) -> LabelPhantom:
+    """Label-map form of the reference's analytic phantom (monte_cu/CBCT_real325.cu:
+    916-928): a water cylinder of radius 10 cm along x with |x| <= 10, holding
+    eight calcium rods (density 1.55) of radius 1.5 cm parallel to x, their axes
+    at radius 5 cm in the y-z plane every 45 degrees.  Voxel centres decide
+    membership; at 0.1 cm the grid is 200^3 over [-10, 10]^3."""
+    n = int(round(20.0 / v))
+    prims = [(_lib.SHAPE_CYLINDER_X, 1, (0.0, 0.0, 0.0), (10.0, 10.0, 10.0))]
+    for k in range(8):
+        a = k * np.pi / 4.0
+        prims.append((_lib.SHAPE_CYLINDER_X, 2, (0.0, 5.0 * np.sin(a), 5.0 * np.cos(a)), (10.0, 1.5, 1.5)))
+    return LabelPhantom(paint((n, n, n), (v, v, v), prims), (v, v, v), list(MATERIALS))

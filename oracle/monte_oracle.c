@@ -490,7 +490,9 @@ void orc_siddon_lengths(const orc_phantom *ph, const double p0[3], const double 
     if (tnext[2] < tnext[a]) a = 2;
     double te = tnext[a] < t1 ? tnext[a] : t1;
     if (te > t) {
-      uint8_t lab = ph->labels[((size_t)idx[2] * ph->ny + idx[1]) * ph->nx + idx[0]];
+      /* labels past the table read as the last label, as in transport */
+      int lab = ph->labels[((size_t)idx[2] * ph->ny + idx[1]) * ph->nx + idx[0]];
+      if (lab > ORC_MAX_MAT - 1) lab = ORC_MAX_MAT - 1;
       len[lab] += (te - t) * L;
       t = te;
     }
@@ -622,13 +624,19 @@ double orc_thomson_mu(double u) {
   return a < 0 ? -m : m;
 }
 
-/* rotate unit direction d by polar cosine c and azimuth phi (direction-cosine form) */
+/* Rotate unit direction d by polar cosine c and azimuth phi (direction-cosine
+ * form).  The new direction is c d + s (cos(phi) e1 + sin(phi) e2) with
+ * s = sqrt(1 - c^2) and the orthonormal pair e1 = (dx dz, dy dz, -r^2)/r,
+ * e2 = (-dy, dx, 0)/r, r = sqrt(dx^2 + dy^2) (accurate near the poles, where
+ * 1 - dz^2 would cancel); only for d on the z axis is the pair the x and y
+ * axes.  The reference updates (theta, phi) angles instead
+ * (monte_cu/CBCT_real325.cu:725-766, without carrying sin_phi_a, quirk Q6). */
 static void rotate_dir(double d[3], double c, double phi) {
   double s = sqrt(fmax(0.0, 1.0 - c * c));
   double cp = cos(phi), sp = sin(phi);
-  double r = sqrt(fmax(0.0, 1.0 - d[2] * d[2]));
+  double r = sqrt(d[0] * d[0] + d[1] * d[1]);
   double n[3];
-  if (r > 1e-5) {
+  if (r > 1e-150) {
     n[0] = c * d[0] + s * (d[0] * d[2] * cp - d[1] * sp) / r;
     n[1] = c * d[1] + s * (d[1] * d[2] * cp + d[0] * sp) / r;
     n[2] = c * d[2] - s * r * cp;
@@ -639,6 +647,17 @@ static void rotate_dir(double d[3], double c, double phi) {
   }
   double nn = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   d[0] = n[0] * nn; d[1] = n[1] * nn; d[2] = n[2] * nn;
+}
+
+void orc_rotate_dir(double d[3], double c, double phi) { rotate_dir(d, c, phi); }
+
+/* Spectrum draw: the first keV bin e in [1, 200] whose cumulative weight
+ * reaches u (a linear scan of the CDF as CBCT_real325.cu:479-486 scans its
+ * table; bin 200 if none does). */
+int orc_spectrum_bin(const double *cdf, double u) {
+  int e = 1;
+  while (e < ORC_NROWS - 1 && cdf[e] < u) ++e;
+  return e;
 }
 
 typedef struct {
@@ -759,12 +778,7 @@ static void mc_items(void *vctx, long i0, long i1) {
       double dir[3] = {(dl[0] * cb - dl[1] * sb) * dn, (dl[0] * sb + dl[1] * cb) * dn, dl[2] * dn};
       double pos[3] = {-g->sod * cb, -g->sod * sb, 0.0};
       double E = cfg->e_kev;
-      if (cfg->spectrum_cdf) {
-        double u = orc_next(&rs);
-        int e = 1;
-        while (e < ORC_NROWS - 1 && cfg->spectrum_cdf[e] < u) ++e;
-        E = e;
-      }
+      if (cfg->spectrum_cdf) E = orc_spectrum_bin(cfg->spectrum_cdf, orc_next(&rs));
       int eb = ebin_of(E);
       int nint = 0;
       for (;;) {
@@ -795,6 +809,7 @@ static void mc_items(void *vctx, long i0, long i1) {
               iy = iy < 0 ? 0 : (iy >= ph->ny ? ph->ny - 1 : iy);
               iz = iz < 0 ? 0 : (iz >= ph->nz ? ph->nz - 1 : iz);
               int lab = ph->labels[((size_t)iz * ph->ny + iy) * ph->nx + ix];
+              if (lab > ORC_MAX_MAT - 1) lab = ORC_MAX_MAT - 1; /* past the table: the last label */
               double mu = mt->mu[lab][eb];
               if (u[2 * h + 1] * mumax < mu) {
                 ended = 1;

@@ -104,7 +104,7 @@ typedef struct orc_phantom {
   int nx, ny, nz;
   double vx, vy, vz;
   double ox, oy, oz;     /* centre offset */
-  int n_labels;          /* labels 0..n_labels-1 */
+  int n_labels;          /* labels 0..n_labels-1; others vacuum, labels > 15 read as 15 */
   int mat_table[ORC_MAX_MAT];   /* table index per label, -1 = vacuum */
   double density[ORC_MAX_MAT];  /* g/cm^3 per label */
 } orc_phantom;
@@ -151,6 +151,10 @@ void orc_mc_run(const orc_geom *g, const orc_phantom *ph, const orc_tables *t,
  * closed forms (Klein-Nishina, Thomson, branching ratios). */
 double orc_kahn_lambda_ratio(double e_kev, const double *u, int *n_used); /* returns lambda'/lambda */
 double orc_thomson_mu(double u);  /* cos(theta) ~ (1+mu^2) */
+/* d <- d turned by polar cosine c and azimuth phi about itself (unit vectors) */
+void orc_rotate_dir(double d[3], double c, double phi);
+/* keV bin of a spectrum draw u: first e in [1, 200] with cdf[e] >= u, else 200 */
+int orc_spectrum_bin(const double *cdf, double u);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,9 @@ def lib():
         L.orc_kahn_lambda_ratio.argtypes = [C.c_double, P, C.POINTER(C.c_int)]
         L.orc_thomson_mu.restype = C.c_double
         L.orc_thomson_mu.argtypes = [C.c_double]
+        L.orc_rotate_dir.argtypes = [P, C.c_double, C.c_double]
+        L.orc_spectrum_bin.restype = C.c_int
+        L.orc_spectrum_bin.argtypes = [P, C.c_double]
         _lib = L
     return _lib
 
@@ -315,3 +318,15 @@ def kahn_ratio(e_kev, uniforms):
 
 def thomson_mu(u):
     return lib().orc_thomson_mu(float(u))
+
+
+def rotate_dir(d, cos_theta, phi):
+    """The oracle's direction update: d turned by polar cosine cos_theta and azimuth phi."""
+    v = np.ascontiguousarray(d, dtype=np.float64).copy()
+    lib().orc_rotate_dir(_p(v), float(cos_theta), float(phi))
+    return v
+
+
+def spectrum_bin(cdf, u):
+    c = np.ascontiguousarray(cdf, dtype=np.float64)
+    return lib().orc_spectrum_bin(_p(c), float(u))

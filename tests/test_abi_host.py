@@ -106,6 +106,8 @@ def _paint_numpy(shape, v, prims):
             m = a * a + b * b + cc * cc <= 1
         elif s == 1:
             m = (a * a + b * b <= 1) & (np.abs(cc) <= 1)
+        elif s == 3:
+            m = (b * b + cc * cc <= 1) & (np.abs(a) <= 1)
         else:
             m = (np.abs(a) <= 1) & (np.abs(b) <= 1) & (np.abs(cc) <= 1)
         lab[m] = l
@@ -116,13 +118,36 @@ def test_phantom_paint_matches_numpy():
     from monte_amd import phantoms
 
     prims = [(0, 1, (0.1, -0.2, 0.05), (1.2, 0.9, 1.1)), (1, 2, (0.4, 0.3, 0.0), (0.3, 0.25, 0.8)),
-             (2, 3, (-0.5, 0.2, -0.3), (0.2, 0.35, 0.15))]
+             (2, 3, (-0.5, 0.2, -0.3), (0.2, 0.35, 0.15)), (3, 4, (0.2, -0.6, 0.5), (0.8, 0.3, 0.45))]
     shape = (23, 29, 31)
     got = phantoms.paint(shape, (0.1, 0.1, 0.1), prims)
     ref = _paint_numpy(shape, 0.1, prims)
     # voxels exactly on a surface may round either way in the two evaluations
     assert np.mean(got != ref) < 1e-3
-    assert np.bincount(got.ravel(), minlength=4)[1:].min() > 0
+    assert np.bincount(got.ravel(), minlength=5)[1:].min() > 0
+
+
+def test_reference_rods_phantom():
+    """The label map of the reference's analytic medium (CBCT_real325.cu:916-928),
+    checked voxel by voxel against the reference's membership tests evaluated
+    at the voxel centres with the same 45-degree rod placement."""
+    from monte_amd import phantoms
+
+    ph = phantoms.reference_rods_phantom(v=0.25)
+    n = ph.labels.shape[0]
+    assert ph.labels.shape == (80, 80, 80) and ph.materials[2][1] == 1.55
+    c = (np.arange(n) - 0.5 * (n - 1)) * 0.25
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    water = (y * y + z * z <= 100) & (np.abs(x) <= 10)
+    rods = np.zeros_like(water)
+    for yc, zc in ((5, 0), (-5, 0), (0, 5), (0, -5)):
+        rods |= (y - yc) ** 2 + (z - zc) ** 2 <= 1.5 ** 2
+    for ang in (np.pi / 4, 3 * np.pi / 4):
+        for sgn in (1, -1):
+            rods |= (y - sgn * 5 * np.sin(ang)) ** 2 + (z - sgn * 5 * np.cos(ang)) ** 2 <= 1.5 ** 2
+    ref = np.where(water, np.where(rods, 2, 1), 0)
+    assert np.mean(ph.labels != ref) < 1e-3
+    assert (ph.labels == 2).sum() > 0.9 * rods[water].sum()
 
 
 def test_phantom_paint_rejects_bad_input():

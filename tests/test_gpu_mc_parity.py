@@ -1,0 +1,176 @@
+"""Monte Carlo parity on the benchmarked paths and at the reference's own geometry.
+
+* Config 1 in full (65^3 phantom, 128^2 detector, 36 views x 1e6 photons):
+  GPU vs the FP64 oracle on the same Philox streams, for the 120 kVp Kramers
+  spectrum with 2.5 mm Al (the energy draw the benchmark times:
+  CBCT_real325.cu:479-486 restated as a CDF search) in both coherent modes, and
+  for the config's own 60 keV mono source.  Per-pixel |z| < 3 of the Poisson
+  bound, >= 95% identical pixels, chi-square reported, interaction totals equal
+  to the few histories that diverge in FP32 vs FP64.
+* The reference geometry (BP3D20 preset: SOD 160, SDD 220, 325^2 detector at
+  0.1 cm, detector index -(int)(10 d - 162.5) of CBCT_real325.cu:558-581 and
+  :809-828), 140 keV mono, undeflected coherent scattering, at most 5
+  interactions, on the label-map form of the reference's water cylinder with
+  eight calcium rods (:916-928):
+    - through air every photon lands in the pixel it was aimed at (the aim of
+      :454-466 and the index of :565-566 are inverse maps);
+    - exact primary vs FP64 Siddon at 1e-5; MC vs the oracle; the primary
+      tally vs Binomial(N0, exp(-L)).
+* Labels 16..255 read as label 15 on the GPU as in the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, oracle_geom, oracle_phantom
+
+pytestmark = pytest.mark.gpu
+
+
+def _tables(oracle):
+    return oracle.tables(np.stack([oracle.read_xcom_csv(os.path.join(ROOT, "monte_amd", "data", f))[0]
+                                   for f in ("xcom2.csv", "Ca.csv")]))
+
+
+def _compare(gpu, ref, st, st_ref, label):
+    for name, a, b in (("primary", gpu[0], ref[0]), ("scatter", gpu[1], ref[1])):
+        assert a.sum() > 0
+        same = np.mean(a == b)
+        var = np.maximum(a + b, 1.0)
+        z = (a - b) / np.sqrt(var)
+        chi2 = float(np.sum((a - b) ** 2 / var))
+        print(f"{label} {name}: identical pixels {same:.4f}  max|z| {np.abs(z).max():.2f}  "
+              f"chi2/dof {chi2 / a.size:.5f}  totals {a.sum():.0f} vs {b.sum():.0f}")
+        assert np.abs(z).max() < 3.0
+        assert same > 0.95
+    for k in ("tallied_primary", "tallied_scatter", "photo", "compton", "coherent", "capped"):
+        assert abs(st[k] - st_ref[k]) <= max(10, 1e-3 * st_ref[k]), (k, st[k], st_ref[k])
+
+
+def _run_pair(ctx, oracle, geom, ph, settings):
+    from monte_amd.mc import mc_project
+    from monte_amd.tables import spectrum_cdf_f32
+
+    prim, scat, st = mc_project(ph, geom, settings, ctx=ctx)
+    cdf = spectrum_cdf_f32(settings.spectrum) if settings.spectrum is not None else None
+    p_ref, s_ref, st_ref = oracle.mc_run(oracle_geom(oracle, geom), oracle_phantom(oracle, ph), _tables(oracle),
+                                         settings.photons_per_view, e_kev=settings.e_kev, spectrum_cdf=cdf,
+                                         max_interactions=settings.max_interactions,
+                                         coherent_mode=settings.coherent, seed=settings.seed, job=settings.job)
+    return (prim.cpu().numpy(), scat.cpu().numpy()), st, (p_ref, s_ref), st_ref
+
+
+@pytest.mark.parametrize("source,coherent", [("spectrum", 0), ("spectrum", 1), ("mono60", 0)])
+def test_mc_config1_full_matches_oracle(ctx, oracle, source, coherent):
+    from monte_amd import geometry as G
+    from monte_amd import phantoms, tables
+    from monte_amd.mc import McSettings
+
+    geom = G.small(n_views=36)
+    ph = phantoms.small_phantom()
+    N = 1_000_000
+    spec = tables.kramers_spectrum(120.0, 2.5) if source == "spectrum" else None
+    s = McSettings(photons_per_view=N, e_kev=60.0, spectrum=spec, coherent=coherent)
+    gpu, st, ref, st_ref = _run_pair(ctx, oracle, geom, ph, s)
+    print("gpu", st)
+    print("cpu", st_ref)
+    assert st["histories"] == st_ref["histories"] == N * geom.n_views
+    assert st["histories"] == st["tallied_primary"] + st["tallied_scatter"] + st["missed"] + st["photo"] + \
+        st["capped"]
+    _compare(gpu, ref, st, st_ref, f"{source}/coh{coherent}")
+    assert st["compton"] > 0 and st["photo"] > 0 and st["coherent"] > 0
+
+
+def _bp3d20_geom(n_views=8, step_deg=45):
+    from monte_amd import geometry as G
+
+    return G.bp3d20_325(n_views=n_views).replace(step_deg=step_deg)
+
+
+def test_bp3d20_air_photons_hit_the_aimed_pixel(ctx):
+    """With nothing to attenuate, photon n aimed at the centre of pixel
+    n mod (nu nv) is tallied in that pixel: the reference's aim (pixel i at
+    u = 16.25 - 0.05 - 0.1 i) and index -(int)(10 d - 162.5) invert each other."""
+    from monte_amd.mc import McSettings, mc_project
+    from monte_amd.phantoms import MATERIALS, LabelPhantom
+
+    geom = _bp3d20_geom(n_views=4)
+    ph = LabelPhantom(np.zeros((20, 20, 20), np.uint8), (0.1, 0.1, 0.1), list(MATERIALS))
+    per_pixel = 3
+    s = McSettings(photons_per_view=per_pixel * geom.nu * geom.nv, e_kev=140.0, coherent=0)
+    prim, scat, st = mc_project(ph, geom, s, ctx=ctx)
+    assert st["missed"] == 0 and st["tallied_scatter"] == 0
+    assert np.all(prim.cpu().numpy() == per_pixel)
+    assert not scat.any()
+
+
+@pytest.fixture(scope="module")
+def rods():
+    from monte_amd import phantoms
+
+    return phantoms.reference_rods_phantom(0.1)
+
+
+def test_bp3d20_primary_exact_matches_siddon(ctx, oracle, rods):
+    from monte_amd.mc import primary_exact
+
+    geom = _bp3d20_geom()
+    got = primary_exact(rods, geom, e_kev=140.0, ctx=ctx).cpu().numpy()
+    ref = oracle.primary_lineint(oracle_geom(oracle, geom), oracle_phantom(oracle, rods), _tables(oracle), 140.0)
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    print("bp3d20 primary: max rel", err.max(), "max L", ref.max())
+    assert ref.max() > 3.0 and err.max() < 1e-5
+
+
+def test_bp3d20_reference_phantom_mc_matches_oracle(ctx, oracle, rods):
+    from monte_amd.mc import McSettings
+
+    geom = _bp3d20_geom()
+    per_pixel = 16
+    s = McSettings(photons_per_view=per_pixel * geom.nu * geom.nv, e_kev=140.0, coherent=0, max_interactions=5)
+    gpu, st, ref, st_ref = _run_pair(ctx, oracle, geom, rods, s)
+    print("gpu", st)
+    print("cpu", st_ref)
+    assert st["histories"] == st_ref["histories"]
+    assert st["capped"] > 0  # the fifth-interaction cap of the reference is reached
+    _compare(gpu, ref, st, st_ref, "bp3d20/140keV")
+    # primary counts are Binomial(N0, exp(-L)) with L the exact line integral
+    L = oracle.primary_lineint(oracle_geom(oracle, geom), oracle_phantom(oracle, rods), _tables(oracle), 140.0)
+    p = np.exp(-L)
+    z = (gpu[0] - per_pixel * p) / np.sqrt(np.maximum(per_pixel * p * (1 - p), 1e-12))
+    inside = p < 0.999
+    chi2 = float(np.mean(z[inside] ** 2))
+    print("primary vs Beer-Lambert chi2/dof", chi2, "pixels", int(inside.sum()))
+    assert inside.sum() > 10000 and 0.95 < chi2 < 1.05
+    assert np.all(gpu[0][L == 0] == per_pixel)  # rays through air only are never absorbed
+
+
+def test_labels_past_table_gpu(ctx, oracle):
+    """A phantom holding labels 16..255 gives the GPU the same exact primary and
+    the same MC tallies as with those voxels at label 15, and matches the oracle."""
+    from monte_amd import geometry as G
+    from monte_amd.mc import McSettings, mc_project, primary_exact
+    from monte_amd.phantoms import LabelPhantom
+
+    rng = np.random.default_rng(21)
+    base = rng.choice([0, 1, 2, 15], size=(30, 34, 38), p=[0.4, 0.3, 0.1, 0.2]).astype(np.uint8)
+    high = base.copy()
+    m = base == 15
+    high[m] = rng.choice([15, 16, 200, 255], size=int(m.sum()))
+    mats = [(-1, 0.0), (0, 1.0), (1, 1.55)] + [(-1, 0.0)] * 12 + [(1, 0.8)]
+    geom = G.small(n_views=4).replace(nu=48, nv=48, du=0.2, dv=0.2)
+    pa = LabelPhantom(base, (0.1, 0.1, 0.1), mats)
+    pb = LabelPhantom(high, (0.1, 0.1, 0.1), mats)
+    la = primary_exact(pa, geom, e_kev=60.0, ctx=ctx)
+    lb = primary_exact(pb, geom, e_kev=60.0, ctx=ctx)
+    assert float(la.max()) > 1.0 and bool((la == lb).all())
+    ref = oracle.primary_lineint(oracle_geom(oracle, geom), oracle_phantom(oracle, pb), _tables(oracle), 60.0)
+    err = np.abs(lb.cpu().numpy() - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    assert err.max() < 1e-5
+    s = McSettings(photons_per_view=100_000, e_kev=60.0)
+    a = mc_project(pa, geom, s, ctx=ctx)
+    b = mc_project(pb, geom, s, ctx=ctx)
+    assert bool((a[0] == b[0]).all()) and bool((a[1] == b[1]).all())
+    gpu, st, ref2, st_ref = _run_pair(ctx, oracle, geom, pb, s)
+    _compare(gpu, ref2, st, st_ref, "labels>15")
