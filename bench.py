@@ -42,6 +42,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=24.0, help="CPU-baseline budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="CPU-baseline threads (0 = every CPU this process may use, capped by OMP_NUM_THREADS)")
+    ap.add_argument("--batched", type=int, default=1, help="1: also time the 64-job batched MC workload")
+    ap.add_argument("--e2e", type=int, default=1, help="1: also time the end-to-end 512^3 / 720-view step")
+    ap.add_argument("--e2e-photons", type=float, default=5e7, help="photons per view of the end-to-end step")
+    ap.add_argument("--e2e-steps", type=int, default=1, help="timed graph replays of the end-to-end step")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: time candidate launch shapes first (the defaults are this workload's best)")
     return ap.parse_args()
@@ -131,8 +135,11 @@ def main():
     mc_s = np.mean([r["mc_ms"] for r in recs]) / 1e3
     mc_k = np.mean([r["mc_kernel_ms"] for r in recs]) / 1e3
     st = recs[-1]["mc_stats"]
-    S = st["steps"] / st["histories"]
-    Cn = st["interactions"] / st["histories"]
+    S_kernel = st["steps"] / st["histories"]
+    # The work model counts Woodcock steps under the global majorant; the kernel
+    # takes fewer (first flights use the majorant of the labels their line
+    # crosses), so S is measured on a subsample run with the global majorant.
+    S, Cn = global_majorant_steps(ctx, ph, geom, settings, labels)
     w_mc = MC_OPS_STEP * S + MC_OPS_INT * Cn + MC_OPS_HIST
     fdk_s = np.mean([r["fdk_ms"] for r in recs]) / 1e3
     updates = geom.nx * geom.ny * geom.nz * geom.n_views
@@ -161,12 +168,14 @@ def main():
         "fdk_batch_ms": fdk_batch,
         "fwd_adjoint_over_fdk": fwd_adj / fdk_batch,
         "mc_steps_per_history": S,
+        "mc_kernel_steps_per_history": S_kernel,
         "mc_interactions_per_history": Cn,
         "roofline": {
             "bound": "valu", "kernel": "mc_transport_kernel",
             "achieved": w_mc * hist / mc_k / 1e12, "peak": PEAK_LANE_OPS / 1e12, "unit": "Tlane-op/s",
             "frac": w_mc * hist / mc_k / PEAK_LANE_OPS, "traffic": profiled_traffic("mc_transport_kernel", a, N),
-            "work_per_unit": "W = 40 S + 150 C + 60 lane-ops per history (S steps, C interactions)",
+            "work_per_unit": "W = 40 S + 150 C + 60 lane-ops per history (S Woodcock steps under the global "
+                             "majorant, C interactions)",
             "kernel_ms": mc_k * 1e3,
         },
         "roofline_fdk_backproject": {
@@ -176,9 +185,147 @@ def main():
         },
         "tuning": tuning,
     }
+    del primary, scatter, stacks, labels
+    torch.cuda.empty_cache()
+    if a.batched:
+        out["batched_mc"] = bench_batched(a, ctx)
+    if a.e2e:
+        out["e2e"] = bench_e2e(a, ctx)
     if a.cpu_seconds > 0:
+        a.ray_brick = tuning["mc_ray_brick"]  # the CPU port runs the same algorithm
         out["cpu_baseline"], out["cpu_baseline_fdk"] = cpu_baselines(a, geom, ph, spec, hist / mc_s, updates / fdk_s)
     print(json.dumps(out))
+
+
+def global_majorant_steps(ctx, ph, geom, settings, labels, photons=2_000_000) -> tuple:
+    """Woodcock steps S and interactions C per history with every flight on the
+    global majorant, on `photons` per view of every view."""
+    import dataclasses
+
+    from monte_amd.mc import mc_project
+
+    brick = ctx.tuning["mc_ray_brick"]
+    ctx.set_tuning(mc_ray_brick=0)
+    try:
+        s = dataclasses.replace(settings, photons_per_view=min(photons, settings.photons_per_view))
+        _, _, st = mc_project(ph, geom, s, labels_dev=labels, ctx=ctx)
+    finally:
+        ctx.set_tuning(mc_ray_brick=brick)
+    return st["steps"] / st["histories"], st["interactions"] / st["histories"]
+
+
+def mc_work(stats) -> tuple:
+    """Steps S and interactions C per history, and W = 40 S + 150 C + 60."""
+    hist = sum(s["histories"] for s in stats)
+    S = sum(s["steps"] for s in stats) / hist
+    Cn = sum(s["interactions"] for s in stats) / hist
+    return S, Cn, MC_OPS_STEP * S + MC_OPS_INT * Cn + MC_OPS_HIST
+
+
+def bench_batched(a, ctx) -> dict:
+    """64 uneven MC jobs in one persistent launch (cubic phantoms of side
+    U[65, 325] voxels, 1e5..1e8 photons per job, mono 40..120 keV, 64..256^2
+    detectors, 1..36 views): the launch timed with its own events, against the
+    same jobs launched one by one."""
+    import numpy as np
+    import torch
+
+    from monte_amd import workloads
+    from monte_amd.mc import device_labels, mc_batch, mc_project
+
+    jobs = workloads.batched_jobs()
+    for j in jobs:
+        j.labels_dev = device_labels(j.phantom, "cuda")
+    hist = sum(j.settings.photons_per_view * j.geom.n_views for j in jobs)
+
+    def zero():
+        for j in jobs:
+            if j.primary is not None:
+                j.primary.zero_()
+                j.scatter.zero_()
+
+    mc_batch(jobs, ctx=ctx)  # warm-up (allocates the tallies)
+    times = []
+    for _ in range(max(1, a.steps)):
+        zero()
+        stats = mc_batch(jobs, ctx=ctx)
+        times.append(stats[0]["kernel_ms"])
+    ms = float(np.mean(times))
+    singles = 0.0
+    for j in jobs:
+        singles += mc_project(j.phantom, j.geom, j.settings, labels_dev=j.labels_dev, ctx=ctx)[2]["kernel_ms"]
+    torch.cuda.synchronize()
+    S_kernel = sum(s["steps"] for s in stats) / sum(s["histories"] for s in stats)
+    brick = ctx.tuning["mc_ray_brick"]
+    ctx.set_tuning(mc_ray_brick=0)  # the work model's steps: every flight on the global majorant
+    try:
+        zero()
+        S, Cn, w = mc_work(mc_batch(jobs, ctx=ctx))
+    finally:
+        ctx.set_tuning(mc_ray_brick=brick)
+    res = {"workload": "64 jobs: cubes of side U[65,325] at 0.5 mm, 1e5..1e8 photons per job, 40..120 keV mono, "
+                       "64..256^2 detectors, 1..36 views (seed 20261202)",
+           "jobs": len(jobs), "histories": hist, "kernel_ms": ms, "histories_per_s": hist / (ms / 1e3),
+           "singles_kernel_ms": singles, "batch_speedup_over_singles": singles / ms,
+           "mc_steps_per_history": S, "mc_kernel_steps_per_history": S_kernel, "mc_interactions_per_history": Cn,
+           "roofline": {"bound": "valu", "kernel": "mc_transport_kernel", "unit": "Tlane-op/s",
+                        "achieved": w * hist / (ms / 1e3) / 1e12, "peak": PEAK_LANE_OPS / 1e12,
+                        "frac": w * hist / (ms / 1e3) / PEAK_LANE_OPS}}
+    del jobs
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_e2e(a, ctx) -> dict:
+    """The end-to-end scan (512^3 phantom at 0.4 mm, 720 views of a 1024 x 768
+    detector, 120 kVp spectrum, 5e7 photons per view): one captured step (MC
+    with scatter, line integrals, FDK, L2 loss against the primary-only FDK,
+    gradient through the adjoint) replayed and timed, plus the FDK and its
+    adjoint timed alone on the step's own operands."""
+    import torch
+
+    from monte_amd import workloads
+    from monte_amd.e2e import EndToEnd
+    from monte_amd.fdk import fdk, fdk_adjoint
+    from monte_amd.mc import McSettings, device_labels, primary_exact
+
+    geom, ph, spec = workloads.e2e_inputs()
+    labels = device_labels(ph, "cuda")
+    N = int(a.e2e_photons)
+    s = McSettings(photons_per_view=N, spectrum=spec, coherent=1, seed=20261202)
+    L = primary_exact(ph, geom, spectrum=spec, labels_dev=labels, ctx=ctx)
+    vol_ref = fdk(L, geom, ctx).contiguous()
+    del L
+    E = EndToEnd(ph, geom, s, vol_ref, labels_dev=labels, ctx=ctx)
+    E.step(graph=False)
+    E.step(graph=True)  # capture
+    torch.cuda.synchronize()
+    t = []
+    for _ in range(max(1, a.e2e_steps)):
+        st = E.step(graph=True)
+        t.append(st["kernel_ms"])
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    fdk(E.lineint, geom, ctx)
+    fdk_adjoint(E.residual, geom, ctx)
+    ev[0].record()
+    fdk(E.lineint, geom, ctx)
+    ev[1].record()
+    ev[2].record()
+    fdk_adjoint(E.residual, geom, ctx)
+    ev[3].record()
+    torch.cuda.synchronize()
+    fdk_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    hist = N * geom.n_views
+    step_ms = sum(t) / len(t)
+    res = {"workload": "512^3 at 0.4 mm, 720 views, 1024x768 detector, 120 kVp, %.0e photons per view, coherent "
+                       "Rayleigh; FDK Shepp-Logan 512^3; loss vs primary-only FDK; hipGraph replay" % N,
+           "step_ms": step_ms, "histories": hist, "histories_per_s_in_step": hist / (step_ms / 1e3),
+           "fdk_ms": fdk_ms, "adjoint_ms": adj_ms, "fdk_gups": geom.nx * geom.ny * geom.nz * geom.n_views / fdk_ms / 1e6,
+           "fwd_adjoint_over_fdk": (fdk_ms + adj_ms) / fdk_ms, "loss": float(E.loss),
+           "mc_kernel_steps_per_history": st["steps"] / st["histories"]}
+    del E, vol_ref, labels
+    torch.cuda.empty_cache()
+    return res
 
 
 def profiled_traffic(kernel, a, photons):
@@ -232,6 +379,7 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
 
     host = host_cpus()
     nthreads = a.cpu_threads or host["share"]
+    ray_brick = a.ray_brick
     og = O.geom(**geom.as_dict())
     oph = O.phantom(ph.labels, *ph.voxel, materials=ph.materials)
     tb = O.tables(tables.reference_tables())
@@ -240,7 +388,8 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
 
     def mc_time(n):
         t0 = time.perf_counter()
-        O.mc_run(og, oph, tb, n, spectrum_cdf=cdf, coherent_mode=1, views=(0, 2), nthreads=nthreads)
+        O.mc_run(og, oph, tb, n, spectrum_cdf=cdf, coherent_mode=1, views=(0, 2), nthreads=nthreads,
+                 ray_brick=ray_brick)
         return time.perf_counter() - t0
 
     # size the sample from a probe, then correct once if the probe (which pays
@@ -253,8 +402,8 @@ def cpu_baselines(a, geom, ph, spec, gpu_hps, gpu_ups):
         n = max(n, int(n * target / max(dt, 1e-3)))
         dt = mc_time(n)
     mc = {"value": 2 * n / dt, "unit": "histories/s", "cores": nthreads, "kind": "port", "host": host,
-          "sample": f"oracle MC (FP64, same physics and Philox streams) on 2 views x {n} photons of the "
-                    f"workload, {dt:.1f} s", "gpu_over_cpu": gpu_hps / (2 * n / dt)}
+          "sample": f"oracle MC (FP64, same physics, Philox streams and majorants) on 2 views x {n} photons "
+                    f"of the workload, {dt:.1f} s", "gpu_over_cpu": gpu_hps / (2 * n / dt)}
 
     def fdk_time(nv):
         g = O.geom(**{**geom.as_dict(), "n_views": nv, "dangle": geom.c().dangle})

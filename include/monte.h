@@ -145,6 +145,10 @@ typedef struct monte_tuning {
   int32_t bp_tile_x, bp_tile_y; /* backprojection block tile (threads) */
   int32_t bp_view_chunk;     /* views per backprojection launch */
   int32_t pj_zchunk;         /* projector voxels per lane along z: 2, 4, 8, or 0 = chosen from nz */
+  int32_t mc_ray_brick;      /* > 0: a photon's first flight takes as Woodcock majorant the largest mu
+                                among the labels its source -> pixel line crosses (found per view and
+                                pixel on bricks of this many voxels a side); 0: the global majorant.
+                                Either is exact; they draw different random numbers. */
 } monte_tuning;
 
 /* ------------------------------------------------------------ context */
@@ -197,6 +201,15 @@ int monte_fdk_adjoint(monte_ctx *ctx, const monte_geom *g, const float *dvol, fl
  * counts per pixel (accumulated into primary/scatter [views][nu][nv]). */
 int monte_mc_project(monte_ctx *ctx, const monte_geom *g, const monte_phantom *ph, const monte_mc_cfg *cfg,
                      double *primary, double *scatter, monte_mc_stats *stats);
+/* The label sets that bound the first flights' Woodcock majorants (tuning
+ * mc_ray_brick), for inspection and tests; no reference counterpart.  sets:
+ * device uint8 [n_views][nv * nu], pixel iv * nu + iu (the aim order); bit l
+ * for label l in 0..3, bit 4 (16) for any label >= 4 or a line that misses the
+ * flight box.  A line's set is the union of the sets of the bricks of brick^3
+ * voxels (each widened by one voxel) it crosses inside the flight box, found
+ * in FP64 by the same operations as the CPU oracle. */
+int monte_mc_first_flight_sets(monte_ctx *ctx, const monte_geom *g, const monte_phantom *ph, int brick,
+                               uint8_t *sets);
 /* Many independent jobs in one persistent launch fed by a device work queue
  * (no reference counterpart).  stats: n_jobs entries or NULL. */
 int monte_mc_batch(monte_ctx *ctx, const monte_mc_job *jobs, int n_jobs, monte_mc_stats *stats);
@@ -247,6 +260,33 @@ int monte_autotune(monte_ctx *ctx, int what, const monte_geom *g, const monte_ph
  * hipGraph owned by the context, and replay it (no reference counterpart).
  * The graph is rebuilt when g, the pointers or batch change. */
 int monte_fdk_graph(monte_ctx *ctx, const monte_geom *g, const float *proj, float *vol, int batch);
+
+/* ------------------------------------------------------------ end to end */
+/* Device buffers of one end-to-end step (caller-owned). */
+typedef struct monte_e2e_buffers {
+  double *primary, *scatter; /* [views][nu][nv] MC tallies, overwritten */
+  float *lineint;            /* [views][nu][nv] line integrals of primary + scatter */
+  float *vol;                /* [nz][ny][nx] FDK of lineint */
+  const float *vol_ref;      /* [nz][ny][nx] target volume */
+  float *residual;           /* [nz][ny][nx] vol - vol_ref */
+  float *grad;               /* [views][nu][nv] gradient of the loss with respect to lineint */
+  double *loss;              /* one double: 0.5 ||vol - vol_ref||^2 */
+} monte_e2e_buffers;
+/* One step of the differentiable scan, chaining what the reference runs as
+ * separate programs joined by files (monte_cu/CBCT_real325.cu:241-300 then
+ * recon/bp3d20_325.cpp:39-175) and adding what it lacks, the loss and its
+ * gradient:
+ *   MC projection of every view (tallies zeroed first), lineint = ln N0 -
+ *   ln clamp(primary + scatter, 1, N0), vol = fdk(lineint), residual = vol -
+ *   vol_ref, loss = 1/2 ||residual||^2 (FP64, fixed summation order),
+ *   grad = fdk^T(residual).
+ * use_graph != 0: the whole step is captured once into a hipGraph owned by the
+ * context and replayed (rebuilt when any argument, the tables or the tuning
+ * change); results are bitwise equal to use_graph == 0.  Device pointers only;
+ * cfg must cover every view.  stats (nullable): the MC counters, with
+ * kernel_ms the device time of the whole step. */
+int monte_e2e_step(monte_ctx *ctx, const monte_geom *g, const monte_phantom *ph, const monte_mc_cfg *cfg,
+                   const monte_e2e_buffers *buf, int use_graph, monte_mc_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -74,10 +74,15 @@ class McJob(C.Structure):
     ]
 
 
+class E2EBuffers(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("primary", "scatter", "lineint", "vol", "vol_ref", "residual", "grad",
+                                          "loss")]
+
+
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "mc_block", "mc_batch", "mc_blocks_per_cu", "bp_zchunk", "bp_tile_x", "bp_tile_y", "bp_view_chunk",
-        "pj_zchunk")]
+        "pj_zchunk", "mc_ray_brick")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -114,6 +119,7 @@ SIGNATURES = {
     "monte_fdk_adjoint": (C.c_int, [_CTX, C.POINTER(Geom), _P, _P, C.c_int]),
     "monte_mc_project": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.POINTER(McCfg), _P, _P,
                                    C.POINTER(McStats)]),
+    "monte_mc_first_flight_sets": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.c_int, _P]),
     "monte_mc_batch": (C.c_int, [_CTX, C.POINTER(McJob), C.c_int, C.POINTER(McStats)]),
     "monte_primary_exact": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.c_double, _P, _P]),
     "monte_counts_to_lineint": (C.c_int, [_CTX, C.POINTER(Geom), _P, C.c_uint64, _P]),
@@ -123,6 +129,8 @@ SIGNATURES = {
     "monte_autotune": (C.c_int, [_CTX, C.c_int, C.POINTER(Geom), C.POINTER(Phantom), C.POINTER(McCfg),
                                  C.POINTER(Tuning)]),
     "monte_fdk_graph": (C.c_int, [_CTX, C.POINTER(Geom), _P, _P, C.c_int]),
+    "monte_e2e_step": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.POINTER(McCfg), C.POINTER(E2EBuffers),
+                                 C.c_int, C.POINTER(McStats)]),
 }
 
 _lib = None

@@ -44,7 +44,7 @@ namespace {
 
 constexpr int kSlotFwd = 0, kSlotAdj = 1, kSlotMc = 2, kSlotConst = 3, kSlotStageA = 4, kSlotStageB = 5,
               kSlotTuneA = 6, kSlotTuneB = 7, kSlotVolT = 8, kSlotQuad = 9,
-              kSlotPlan = 10;
+              kSlotPlan = 10, kSlotE2E = 11, kSlotRay = 12, kSlotE2ERay = 13;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -260,6 +260,8 @@ struct HostJob {
   McTables tables{};
   std::vector<float> cdf;       // empty = mono
   std::vector<float2> view_cs;  // all views of the geometry
+  std::vector<double> cs64;     // FP64 (cos, sin) of all views, for the first-flight label sets
+  RayGrid ray{};                // first-flight label-set geometry (box and bricks filled in by prepare_mc)
 };
 
 int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const monte_mc_cfg* cfg, HostJob& hj,
@@ -333,11 +335,25 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
     hj.cdf[kRows - 1] = 1.0f;
   }
   hj.view_cs.resize(g->n_views);
+  hj.cs64.resize(2 * (size_t)g->n_views);
   for (int k = 0; k < g->n_views; ++k) {
     const double b = g->preset == MONTE_PRESET_BP3D20 ? 3.14159265358979323846 * (k * g->step_deg) / 180.0
                                                       : g->angle0 + k * g->dangle;
-    hj.view_cs[k] = make_float2((float)std::cos(b), (float)std::sin(b));
+    hj.cs64[2 * k] = std::cos(b);
+    hj.cs64[2 * k + 1] = std::sin(b);
+    hj.view_cs[k] = make_float2((float)hj.cs64[2 * k], (float)hj.cs64[2 * k + 1]);
   }
+  RayGrid& R = hj.ray;
+  if (g->preset == MONTE_PRESET_BP3D20) {
+    R.un0 = 0.5 * g->nu; R.u_step = -g->du; R.vn0 = 0.5 * g->nv; R.v_step = -g->dv;
+  } else {
+    R.un0 = 0.5 * g->nu - g->u_off / g->du; R.u_step = g->du;
+    R.vn0 = 0.5 * g->nv - g->v_off / g->dv; R.v_step = g->dv;
+  }
+  R.sod = g->sod;
+  R.xd = g->sdd - g->sod;
+  R.nu = g->nu;
+  R.nv = g->nv;
   McJob& J = hj.d;
   J.labels = ph->labels;
   J.nx = ph->nx; J.ny = ph->ny; J.nz = ph->nz;
@@ -375,13 +391,6 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
   return MONTE_OK;
 }
 
-// Where a launch's job descriptors, tables, CDFs, view tables, queue, counters
-// and flight boxes sit in its device buffer.
-struct McLayout {
-  size_t jobs_off = 0, tab_off = 0, q_off = 0, cnt_off = 0, box_off = 0, total = 0;
-  std::vector<size_t> cdf_off, cs_off;
-};
-
 McLayout mc_layout(const std::vector<HostJob>& hj) {
   const int n = (int)hj.size();
   McLayout L;
@@ -401,17 +410,55 @@ McLayout mc_layout(const std::vector<HostJob>& hj) {
   return L;
 }
 
-// Lay out jobs, tables, CDFs and view tables in a device buffer and launch.
-// The buffer is the context's MC workspace, or `base` (at least
-// mc_layout(hj).total bytes) when the caller provides one.
-int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blocks_per_cu, float* ms_out,
-           std::vector<McCounters>* counters_out, char* base = nullptr) {
+// First-flight label sets of every job (tuning mc_ray_brick > 0) in workspace
+// slot `slot`: the FP64 view tables, the brick sets and the per-view,
+// per-pixel sets; each job's descriptor then points at its sets.  The flight
+// boxes must be final.
+int prepare_ray_sets(monte_ctx* c, std::vector<HostJob>& hj, int slot) {
+  const int b = c->tuning.mc_ray_brick;
+  const int n = (int)hj.size();
+  std::vector<size_t> cs_off(n), br_off(n), set_off(n);
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    HostJob& h = hj[i];
+    const McJob& J = h.d;
+    RayGrid& R = h.ray;
+    const int dims[3] = {J.nx, J.ny, J.nz};
+    for (int a = 0; a < 3; ++a) {
+      R.org[a] = h.org[a];
+      R.bw[a] = b * h.vox[a];
+      R.nb[a] = (dims[a] + b - 1) / b;
+    }
+    cs_off[i] = off; off = align_up(off + sizeof(double) * h.cs64.size(), 256);
+    br_off[i] = off; off = align_up(off + (size_t)R.nb[0] * R.nb[1] * R.nb[2], 256);
+    set_off[i] = off; off = align_up(off + (size_t)J.n_views_sim * J.nu * J.nv, 256);
+  }
+  char* base = (char*)workspace(c, slot, off);
+  if (!base) return fail(c, MONTE_ERR_NOMEM, "first-flight label set allocation failed");
+  for (int i = 0; i < n; ++i) {
+    HostJob& h = hj[i];
+    McJob& J = h.d;
+    MONTE_HIP(c, hipMemcpyAsync(base + cs_off[i], h.cs64.data(), sizeof(double) * h.cs64.size(),
+                                hipMemcpyHostToDevice, c->stream));
+    uint8_t* bricks = (uint8_t*)(base + br_off[i]);
+    uint8_t* sets = (uint8_t*)(base + set_off[i]);
+    MONTE_HIP(c, launch_brick_sets(J.labels, J.nx, J.ny, J.nz, b, h.ray.nb, bricks, c->stream));
+    MONTE_HIP(c, launch_ray_sets(h.ray, (const double*)(base + cs_off[i]), J.view_begin, J.n_views_sim, bricks, sets,
+                                 c->stream));
+    J.ray_set = sets;
+  }
+  return MONTE_OK;
+}
+
+// Lays out the jobs in `base` (at least mc_layout(hj).total bytes; the
+// context's MC workspace when null), computes each job's flight box and, with
+// tuning mc_ray_brick > 0 and ray_slot >= 0, its first-flight label sets in
+// workspace slot ray_slot, and uploads everything; returns once the uploads
+// are complete.
+int prepare_mc(monte_ctx* c, std::vector<HostJob>& hj, int block, int blocks_per_cu, char* base, McPrepared& P,
+               int ray_slot) {
   const int n = (int)hj.size();
   const McLayout Lo = mc_layout(hj);
-  const size_t jobs_off = Lo.jobs_off, tab_off = Lo.tab_off, q_off = Lo.q_off, cnt_off = Lo.cnt_off,
-               box_off = Lo.box_off;
-  const std::vector<size_t>& cdf_off = Lo.cdf_off;
-  const std::vector<size_t>& cs_off = Lo.cs_off;
   if (!base) base = (char*)workspace(c, kSlotMc, Lo.total);
   if (!base) return fail(c, MONTE_ERR_NOMEM, "MC descriptor allocation failed");
 
@@ -425,7 +472,7 @@ int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blo
       b[0] = b[1] = b[2] = INT_MAX;
       b[3] = b[4] = b[5] = -1;
     }
-    int* dbox = (int*)(base + box_off);
+    int* dbox = (int*)(base + Lo.box_off);
     MONTE_HIP(c, hipMemcpyAsync(dbox, box.data(), sizeof(int) * box.size(), hipMemcpyHostToDevice, c->stream));
     for (int i = 0; i < n; ++i) {
       uint32_t solid = 0;
@@ -439,23 +486,40 @@ int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blo
     MONTE_HIP(c, hipStreamSynchronize(c->stream));
     for (int i = 0; i < n; ++i) {
       const int* b = box.data() + 8 * i;
-      if (b[3] < 0) continue;  // nothing attenuates: keep the grid box
       McJob& J = hj[i].d;
       const double* o = hj[i].org;
       const double* v = hj[i].vox;
-      J.bx0 = (float)(o[0] + b[0] * v[0]); J.bx1 = (float)(o[0] + (b[3] + 1) * v[0]);
-      J.by0 = (float)(o[1] + b[1] * v[1]); J.by1 = (float)(o[1] + (b[4] + 1) * v[1]);
-      J.bz0 = (float)(o[2] + b[2] * v[2]); J.bz1 = (float)(o[2] + (b[5] + 1) * v[2]);
+      RayGrid& R = hj[i].ray;
+      const int dims[3] = {J.nx, J.ny, J.nz};
+      if (b[3] < 0) {  // nothing attenuates: keep the grid box
+        for (int a = 0; a < 3; ++a) {
+          R.c0[a] = o[a];
+          R.c1[a] = o[a] + dims[a] * v[a];
+        }
+        continue;
+      }
+      for (int a = 0; a < 3; ++a) {
+        R.c0[a] = o[a] + b[a] * v[a];
+        R.c1[a] = o[a] + (b[3 + a] + 1) * v[a];
+      }
+      J.bx0 = (float)R.c0[0]; J.bx1 = (float)R.c1[0];
+      J.by0 = (float)R.c0[1]; J.by1 = (float)R.c1[1];
+      J.bz0 = (float)R.c0[2]; J.bz1 = (float)R.c1[2];
     }
+  }
+  for (int i = 0; i < n; ++i) hj[i].d.ray_set = nullptr;
+  if (ray_slot >= 0 && c->tuning.mc_ray_brick > 0) {
+    const int rc = prepare_ray_sets(c, hj, ray_slot);
+    if (rc) return rc;
   }
 
   uint64_t units = 0;
   std::vector<McJob> jobs(n);
   for (int i = 0; i < n; ++i) {
     McJob J = hj[i].d;
-    J.tables = (const McTables*)(base + tab_off) + i;
-    J.spectrum_cdf = hj[i].cdf.empty() ? nullptr : (const float*)(base + cdf_off[i]);
-    J.view_cs = (const float2*)(base + cs_off[i]);
+    J.tables = (const McTables*)(base + Lo.tab_off) + i;
+    J.spectrum_cdf = hj[i].cdf.empty() ? nullptr : (const float*)(base + Lo.cdf_off[i]);
+    J.view_cs = (const float2*)(base + Lo.cs_off[i]);
     J.unit_begin = units;
     units += J.unit_count;
     jobs[i] = J;
@@ -464,27 +528,54 @@ int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blo
   const int grid_waves = std::max(1, c->n_cu * blocks_per_cu) * (block / 64);
   if (units > 0xFFFFFFFFull - (uint64_t)grid_waves)
     return fail(c, MONTE_ERR_INVALID, "too many work units in one launch (raise the batch)");
-  MONTE_HIP(c, hipMemcpyAsync(base + jobs_off, jobs.data(), sizeof(McJob) * n, hipMemcpyHostToDevice, c->stream));
+  MONTE_HIP(c, hipMemcpyAsync(base + Lo.jobs_off, jobs.data(), sizeof(McJob) * n, hipMemcpyHostToDevice, c->stream));
   for (int i = 0; i < n; ++i) {
-    MONTE_HIP(c, hipMemcpyAsync(base + tab_off + sizeof(McTables) * i, &hj[i].tables, sizeof(McTables),
+    MONTE_HIP(c, hipMemcpyAsync(base + Lo.tab_off + sizeof(McTables) * i, &hj[i].tables, sizeof(McTables),
                                 hipMemcpyHostToDevice, c->stream));
     if (!hj[i].cdf.empty())
-      MONTE_HIP(c, hipMemcpyAsync(base + cdf_off[i], hj[i].cdf.data(), sizeof(float) * hj[i].cdf.size(),
+      MONTE_HIP(c, hipMemcpyAsync(base + Lo.cdf_off[i], hj[i].cdf.data(), sizeof(float) * hj[i].cdf.size(),
                                   hipMemcpyHostToDevice, c->stream));
-    MONTE_HIP(c, hipMemcpyAsync(base + cs_off[i], hj[i].view_cs.data(), sizeof(float2) * hj[i].view_cs.size(),
+    MONTE_HIP(c, hipMemcpyAsync(base + Lo.cs_off[i], hj[i].view_cs.data(), sizeof(float2) * hj[i].view_cs.size(),
                                 hipMemcpyHostToDevice, c->stream));
   }
-  unsigned int* queue = (unsigned int*)(base + q_off);
-  McCounters* counters = (McCounters*)(base + cnt_off);
-  MONTE_HIP(c, launch_mc_reset(queue, counters, n, c->stream));
-  MONTE_HIP(c, hipEventRecord(c->ev0, c->stream));
+  MONTE_HIP(c, hipStreamSynchronize(c->stream));  // the host copies above must outlive the uploads
+  P.base = base;
+  P.L = Lo;
+  P.n = n;
+  P.units = units;
+  return MONTE_OK;
+}
+
+// Resets the work queue and counters, then launches the transport (with
+// `start` recorded between the two when given, so it times the transport alone).
+hipError_t launch_prepared(monte_ctx* c, const McPrepared& P, int batch, int block, int blocks_per_cu,
+                           hipEvent_t start = nullptr) {
+  hipError_t e = launch_mc_reset(P.queue(), P.counters(), P.n, c->stream);
+  if (e != hipSuccess) return e;
+  if (start && (e = hipEventRecord(start, c->stream)) != hipSuccess) return e;
   const int grid = std::max(1, c->n_cu * blocks_per_cu);
-  hipError_t le = launch_mc_transport((const McJob*)(base + jobs_off), n, units, queue, counters, batch, block, grid,
-                                      c->stream);
-  MONTE_HIP(c, hipEventRecord(c->ev1, c->stream));
-  std::vector<McCounters> cnt(n);
-  MONTE_HIP(c, hipMemcpyAsync(cnt.data(), counters, sizeof(McCounters) * n, hipMemcpyDeviceToHost, c->stream));
+  return launch_mc_transport((const McJob*)(P.base + P.L.jobs_off), P.n, P.units, P.queue(), P.counters(), batch,
+                             block, grid, c->stream);
+}
+
+int read_counters(monte_ctx* c, const McPrepared& P, std::vector<McCounters>& cnt) {
+  cnt.resize(P.n);
+  MONTE_HIP(c, hipMemcpyAsync(cnt.data(), P.counters(), sizeof(McCounters) * P.n, hipMemcpyDeviceToHost, c->stream));
   MONTE_HIP(c, hipStreamSynchronize(c->stream));
+  return MONTE_OK;
+}
+
+// Prepares and runs one launch in a device buffer: the context's MC workspace,
+// or `base` (at least mc_layout(hj).total bytes) when the caller provides one.
+int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blocks_per_cu, float* ms_out,
+           std::vector<McCounters>* counters_out, char* base = nullptr, int ray_slot = kSlotRay) {
+  McPrepared P;
+  int rc = prepare_mc(c, hj, block, blocks_per_cu, base, P, ray_slot);
+  if (rc) return rc;
+  const hipError_t le = launch_prepared(c, P, batch, block, blocks_per_cu, c->ev0);
+  MONTE_HIP(c, hipEventRecord(c->ev1, c->stream));
+  std::vector<McCounters> cnt;
+  if ((rc = read_counters(c, P, cnt))) return rc;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
   if (le != hipSuccess) return fail(c, MONTE_ERR_HIP, std::string("transport launch: ") + hipGetErrorString(le));
@@ -501,6 +592,48 @@ void fill_stats(monte_mc_stats* s, const McCounters& k, double ms) {
   s->kernel_ms = ms;
 }
 
+// Key of an MC job's prepared descriptors: everything build_job reads.
+std::vector<double> mc_job_key(const monte_ctx* c, const monte_geom& g, const monte_phantom& ph,
+                               const monte_mc_cfg& cfg) {
+  std::vector<double> k = geom_key(g);
+  const double vals[] = {(double)(uintptr_t)ph.labels, (double)ph.nx, (double)ph.ny, (double)ph.nz,
+                         (double)ph.n_labels, ph.vx, ph.vy, ph.vz, ph.ox, ph.oy, ph.oz,
+                         (double)cfg.photons_per_view, cfg.e_kev, (double)cfg.max_interactions,
+                         (double)cfg.coherent_mode, (double)(cfg.seed & 0xFFFFFFFFull), (double)(cfg.seed >> 32),
+                         (double)cfg.job, (double)cfg.view_begin, (double)cfg.view_end, (double)c->tables_gen,
+                         (double)c->tuning.mc_batch, (double)c->tuning.mc_block, (double)c->tuning.mc_blocks_per_cu,
+                         (double)c->tuning.mc_ray_brick};
+  k.insert(k.end(), std::begin(vals), std::end(vals));
+  for (int l = 0; l < kMaxLabels; ++l) {
+    k.push_back((double)ph.mat_table[l]);
+    k.push_back(ph.density[l]);
+  }
+  k.push_back(cfg.spectrum ? 1.0 : 0.0);
+  if (cfg.spectrum) k.insert(k.end(), cfg.spectrum, cfg.spectrum + kRows);
+  return k;
+}
+
+// The launches of one end-to-end step, in stream order (no host syncs, no
+// allocation once the workspaces have their sizes): zero the tallies, MC
+// transport, counts -> line integrals of primary + scatter, FDK, residual and
+// loss, adjoint of the FDK applied to the residual.
+int e2e_launches(monte_ctx* c, const monte_geom& g, const monte_e2e_buffers& b, uint64_t ppv, double* partial) {
+  const size_t np = proj_elems(g), nvx = vol_elems(g);
+  MONTE_HIP(c, launch_fill((float*)b.primary, 0.f, 2 * np, c->stream));
+  MONTE_HIP(c, launch_fill((float*)b.scatter, 0.f, 2 * np, c->stream));
+  const monte_tuning& t = c->tuning;
+  MONTE_HIP(c, launch_prepared(c, c->e2e_mc, 64 * t.mc_batch, t.mc_block, t.mc_blocks_per_cu));
+  MONTE_HIP(c, launch_counts_to_lineint(b.primary, b.scatter, b.lineint, g.n_views, g.nu, g.nv, ppv, c->stream));
+  float* pad = (float*)c->ws[kSlotFwd];
+  int rc = run_filter_padded(c, g, b.lineint, pad);
+  if (!rc) rc = run_backproject_padded(c, g, pad, b.vol, 0);
+  if (rc) return rc;
+  MONTE_HIP(c, launch_residual_loss(b.vol, b.vol_ref, b.residual, nvx, partial, b.loss, c->stream));
+  float* apad = (float*)c->ws[kSlotAdj];
+  if ((rc = run_project_padded(c, g, b.residual, apad))) return rc;
+  return run_filter_T_padded(c, g, apad, b.grad);
+}
+
 monte_tuning default_tuning() {
   monte_tuning t{};
   t.mc_block = 256;
@@ -511,6 +644,7 @@ monte_tuning default_tuning() {
   t.bp_tile_y = 32;
   t.bp_view_chunk = 0;
   t.pj_zchunk = 0;
+  t.mc_ray_brick = 4;  // 2.45 Woodcock steps per history at 325^3 / 512^2 (7.56 with the global majorant)
   return t;
 }
 
@@ -567,6 +701,8 @@ void monte_destroy(monte_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->e2e_exec) (void)hipGraphExecDestroy(c->e2e_exec);
+  if (c->e2e_graph) (void)hipGraphDestroy(c->e2e_graph);
   for (int i = 0; i < monte_ctx::kSlots; ++i)
     if (c->ws[i]) (void)hipFree(c->ws[i]);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -611,6 +747,7 @@ int monte_set_tuning(monte_ctx* c, const monte_tuning* in) {
   if (t.bp_view_chunk < 0) return fail(c, MONTE_ERR_INVALID, "bp_view_chunk");
   if (t.pj_zchunk != 0 && t.pj_zchunk != 2 && t.pj_zchunk != 4 && t.pj_zchunk != 8)
     return fail(c, MONTE_ERR_INVALID, "pj_zchunk");
+  if (t.mc_ray_brick < 0 || t.mc_ray_brick > 64) return fail(c, MONTE_ERR_INVALID, "mc_ray_brick");
   c->tuning = t;
   return MONTE_OK;
 }
@@ -653,6 +790,7 @@ int monte_load_tables(monte_ctx* c, const double* tab, int n_tables) {
     if (!(tab[i] >= 0)) return fail(c, MONTE_ERR_INVALID, "attenuation tables must be >= 0");
   c->tables.assign(tab, tab + (size_t)n_tables * 4 * kRows);
   c->n_tables = n_tables;
+  c->tables_gen += 1;
   return MONTE_OK;
 }
 
@@ -830,6 +968,32 @@ int monte_mc_project(monte_ctx* c, const monte_geom* g, const monte_phantom* ph,
   return st.back(scatter, ds, np * sizeof(double));
 }
 
+int monte_mc_first_flight_sets(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, int brick,
+                               uint8_t* sets) {
+  if (!c) return MONTE_ERR_INVALID;
+  if (!g || !ph || !sets) return fail(c, MONTE_ERR_INVALID, "null argument");
+  if (brick < 1 || brick > 64) return fail(c, MONTE_ERR_INVALID, "brick must be in [1, 64]");
+  if (c->host_ptrs) return fail(c, MONTE_ERR_UNSUPPORTED, "monte_mc_first_flight_sets takes device pointers only");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  monte_mc_cfg cfg{};
+  cfg.photons_per_view = 1;
+  cfg.e_kev = 60.0;
+  std::vector<HostJob> hj(1);
+  int rc = build_job(c, g, ph, &cfg, hj[0], 64);
+  if (rc) return rc;
+  // the flight box as the transport computes it, then the sets with `brick`
+  const monte_tuning saved = c->tuning;
+  c->tuning.mc_ray_brick = brick;
+  McPrepared P;
+  rc = prepare_mc(c, hj, saved.mc_block, saved.mc_blocks_per_cu, nullptr, P, kSlotRay);
+  c->tuning = saved;
+  if (rc) return rc;
+  MONTE_HIP(c, hipMemcpyAsync(sets, hj[0].d.ray_set, (size_t)g->n_views * g->nu * g->nv, hipMemcpyDeviceToDevice,
+                              c->stream));
+  MONTE_HIP(c, hipStreamSynchronize(c->stream));
+  return MONTE_OK;
+}
+
 int monte_primary_exact(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, double e_kev,
                         const double* spectrum, float* lineint) {
   if (!c) return MONTE_ERR_INVALID;
@@ -912,7 +1076,7 @@ int monte_counts_to_lineint(monte_ctx* c, const monte_geom* g, const double* cou
   const double* din = (const double*)st.in(kSlotStageA, counts, np * sizeof(double));
   float* dout = (float*)st.out(kSlotStageB, lineint, np * sizeof(float), false);
   if (st.code) return st.code;
-  MONTE_HIP(c, launch_counts_to_lineint(din, dout, g->n_views, g->nu, g->nv, photons_per_view, c->stream));
+  MONTE_HIP(c, launch_counts_to_lineint(din, nullptr, dout, g->n_views, g->nu, g->nv, photons_per_view, c->stream));
   return st.back(lineint, dout, np * sizeof(float));
 }
 
@@ -1117,6 +1281,96 @@ int monte_fdk_graph(monte_ctx* c, const monte_geom* g, const float* proj, float*
     c->graph_key = key;
   }
   MONTE_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+  return MONTE_OK;
+}
+
+// ------------------------------------------------------------ end to end
+int monte_e2e_step(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const monte_mc_cfg* cfg,
+                   const monte_e2e_buffers* buf, int use_graph, monte_mc_stats* stats) {
+  if (!c) return MONTE_ERR_INVALID;
+  int rc = check_geom(c, g, true);
+  if (rc) return rc;
+  if (!ph || !cfg || !buf) return fail(c, MONTE_ERR_INVALID, "null argument");
+  const monte_e2e_buffers& b = *buf;
+  if (!b.primary || !b.scatter || !b.lineint || !b.vol || !b.vol_ref || !b.residual || !b.grad || !b.loss)
+    return fail(c, MONTE_ERR_INVALID, "null buffer");
+  if (c->host_ptrs) return fail(c, MONTE_ERR_UNSUPPORTED, "monte_e2e_step takes device pointers only");
+  if (use_graph && c->stream == nullptr)
+    return fail(c, MONTE_ERR_INVALID, "graph capture needs a created stream, not the null stream");
+  if (!((cfg->view_begin == 0 && (cfg->view_end == 0 || cfg->view_end == g->n_views))))
+    return fail(c, MONTE_ERR_INVALID, "the end-to-end step simulates every view of the scan");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  const int batch = 64 * c->tuning.mc_batch;
+
+  // MC descriptors, prepared once per job in their own slot
+  std::vector<double> mkey = mc_job_key(c, *g, *ph, *cfg);
+  mkey.push_back((double)(uintptr_t)b.primary);  // the descriptors hold the tally pointers
+  mkey.push_back((double)(uintptr_t)b.scatter);
+  if (mkey != c->e2e_mc_key || !c->e2e_mc.base) {
+    c->e2e_mc_key.clear();
+    c->e2e_graph_key.clear();
+    std::vector<HostJob> hj(1);
+    if ((rc = build_job(c, g, ph, cfg, hj[0], batch))) return rc;
+    hj[0].d.primary = b.primary;
+    hj[0].d.scatter = b.scatter;
+    const size_t need = mc_layout(hj).total + kLossBlocks * sizeof(double);
+    char* base = (char*)workspace(c, kSlotE2E, need);
+    if (!base) return fail(c, MONTE_ERR_NOMEM, "end-to-end descriptor allocation failed");
+    if ((rc = prepare_mc(c, hj, c->tuning.mc_block, c->tuning.mc_blocks_per_cu, base, c->e2e_mc, kSlotE2ERay)))
+      return rc;
+    c->e2e_mc_key = mkey;
+  }
+  double* partial = (double*)(c->e2e_mc.base + c->e2e_mc.L.total);
+
+  // every workspace the launches touch exists at its size before any capture
+  const float* taps;
+  const BpView* views;
+  if ((rc = ensure_constants(c, *g, &taps, &views))) return rc;
+  const BpGrid grid = compute_bp_grid(*g);
+  const uint32_t *plan_offs, *plan_list;
+  if ((rc = ensure_projector_plan(c, *g, views, grid, &plan_offs, &plan_list))) return rc;
+  if (!workspace(c, kSlotFwd, padded_elems(*g) * sizeof(float)) ||
+      !workspace(c, kSlotQuad, padded_elems(*g) * sizeof(float4)) ||
+      !workspace(c, kSlotAdj, padded_elems(*g) * sizeof(float)) ||
+      !workspace(c, kSlotVolT, vol_elems(*g) * sizeof(float)))
+    return fail(c, MONTE_ERR_NOMEM, "end-to-end workspace allocation failed");
+
+  MONTE_HIP(c, hipEventRecord(c->ev0, c->stream));
+  if (!use_graph) {
+    if ((rc = e2e_launches(c, *g, b, cfg->photons_per_view, partial))) return rc;
+  } else {
+    std::vector<double> key = mkey;
+    const void* ptrs[] = {b.primary, b.scatter, b.lineint, b.vol, b.vol_ref, b.residual, b.grad, b.loss};
+    for (const void* p : ptrs) key.push_back((double)(uintptr_t)p);
+    for (int slot : {kSlotFwd, kSlotQuad, kSlotAdj, kSlotVolT, kSlotPlan, kSlotConst, kSlotE2E, kSlotE2ERay})
+      key.push_back((double)(uintptr_t)c->ws[slot]);
+    const monte_tuning& t = c->tuning;
+    for (double v : {(double)t.bp_zchunk, (double)t.bp_tile_x, (double)t.bp_view_chunk, (double)t.pj_zchunk})
+      key.push_back(v);
+    if (key != c->e2e_graph_key || !c->e2e_exec) {
+      if (c->e2e_exec) { (void)hipGraphExecDestroy(c->e2e_exec); c->e2e_exec = nullptr; }
+      if (c->e2e_graph) { (void)hipGraphDestroy(c->e2e_graph); c->e2e_graph = nullptr; }
+      c->e2e_graph_key.clear();
+      MONTE_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int crc = e2e_launches(c, *g, b, cfg->photons_per_view, partial);
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+      if (crc) { if (graph) (void)hipGraphDestroy(graph); return crc; }
+      if (ce != hipSuccess) return fail(c, MONTE_ERR_HIP, std::string("end capture: ") + hipGetErrorString(ce));
+      c->e2e_graph = graph;
+      MONTE_HIP(c, hipGraphInstantiate(&c->e2e_exec, c->e2e_graph, nullptr, nullptr, 0));
+      c->e2e_graph_key = key;
+    }
+    MONTE_HIP(c, hipGraphLaunch(c->e2e_exec, c->stream));
+  }
+  MONTE_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (stats) {
+    std::vector<McCounters> cnt;
+    if ((rc = read_counters(c, c->e2e_mc, cnt))) return rc;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    fill_stats(stats, cnt[0], ms);
+  }
   return MONTE_OK;
 }
 

@@ -1126,6 +1126,46 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
   }
 }
 
+// ------------------------------------------------------------ L2 loss
+// out = a - b and the FP64 sum of out^2: each block sums a fixed grid-stride
+// share in a fixed order and reduces it in a fixed tree, so the result does
+// not depend on scheduling.
+__global__ __launch_bounds__(256) void residual_partial_kernel(const float* __restrict__ a,
+                                                               const float* __restrict__ b, float* __restrict__ out,
+                                                               size_t n, double* __restrict__ partial) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const float r = a[i] - b[i];
+    out[i] = r;
+    acc = fma((double)r, (double)r, acc);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void loss_total_kernel(const double* __restrict__ partial, int n,
+                                                         double* __restrict__ loss) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += partial[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) *loss = 0.5 * ((red[0] + red[1]) + (red[2] + red[3]));
+}
+
+hipError_t launch_residual_loss(const float* a, const float* b, float* out, size_t n, double* partial, double* loss,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(residual_partial_kernel, dim3(kLossBlocks), dim3(256), 0, s, a, b, out, n, partial);
+  hipLaunchKernelGGL(loss_total_kernel, dim3(1), dim3(256), 0, s, partial, kLossBlocks, loss);
+  return hipGetLastError();
+}
+
 // vol [z][y][x] -> volT [y][x][z] through 32 x 32 LDS tiles (per y).
 __global__ __launch_bounds__(256) void transpose_zx_kernel(const float* __restrict__ vol, float* __restrict__ volT,
                                                            int nx, int ny, int nz) {

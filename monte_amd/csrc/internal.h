@@ -109,6 +109,22 @@ struct McJob {
   uint32_t batches_per_view;     // ceil(photons_per_view / batch)
   uint64_t unit_begin;           // first global work unit of this job
   uint64_t unit_count;
+  const uint8_t* ray_set;        // [n_views_sim][nu nv] first-flight label sets (aim order), or null
+};
+
+// FP64 geometry of the first-flight label sets (ray_sets_kernel).  A photon's
+// first flight runs from the source towards the centre of its pixel, so the
+// labels that segment crosses bound the attenuation it meets, and the largest
+// mu among them is a valid Woodcock majorant for that flight.  Sets are bit
+// masks: bit l for labels 0..3, bit 4 for any label >= 4 (then the flight
+// takes the global majorant).
+struct RayGrid {
+  double un0, u_step, vn0, v_step;  // pixel centre: y' = (iu + 1/2 - un0) u_step, z' likewise
+  double sod, xd;                   // source distance, detector plane x' = xd
+  double c0[3], c1[3];              // flight box (bounding box of the attenuating voxels)
+  double org[3], bw[3];             // brick grid corner and brick size
+  int nb[3];                        // bricks per axis
+  int nu, nv;
 };
 
 // FP64 ray geometry of the exact primary estimator.
@@ -122,6 +138,24 @@ struct PrimaryGeom {
 struct McCounters {  // device-side statistics, one per job
   unsigned long long histories, steps, interactions, photo, coherent, compton;
   unsigned long long tallied_primary, tallied_scatter, missed, capped;
+};
+
+// Where a launch's job descriptors, tables, CDFs, view tables, queue, counters
+// and flight boxes sit in its device buffer.
+struct McLayout {
+  size_t jobs_off = 0, tab_off = 0, q_off = 0, cnt_off = 0, box_off = 0, total = 0;
+  std::vector<size_t> cdf_off, cs_off;
+};
+
+// A transport launch whose descriptors are in place in device memory;
+// launching it is stream-ordered work only (capturable into a graph).
+struct McPrepared {
+  char* base = nullptr;
+  McLayout L;
+  int n = 0;
+  uint64_t units = 0;
+  unsigned int* queue() const { return (unsigned int*)(base + L.q_off); }
+  McCounters* counters() const { return (McCounters*)(base + L.cnt_off); }
 };
 
 }  // namespace monte
@@ -149,7 +183,9 @@ struct monte_ctx {
   //   8 z-fastest copy of the volume for the matched projector
   //   9 tap quads of the forward filtered stack (backprojection input)
   //  10 column lists of the matched projector (per geometry)
-  static constexpr int kSlots = 11;
+  //  11 end-to-end step: MC descriptors and loss partial sums
+  //  12 first-flight label sets of MC launches   13 those of the end-to-end step
+  static constexpr int kSlots = 14;
   void* ws[kSlots] = {};
   size_t ws_size[kSlots] = {};
 
@@ -165,6 +201,16 @@ struct monte_ctx {
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
   std::vector<double> graph_key;
+
+  // bumped by every monte_load_tables (keys of cached MC descriptors)
+  uint64_t tables_gen = 0;
+
+  // end-to-end step: prepared MC launch (keyed by job), and its graph
+  monte::McPrepared e2e_mc;
+  std::vector<double> e2e_mc_key;
+  hipGraphExec_t e2e_exec = nullptr;
+  hipGraph_t e2e_graph = nullptr;
+  std::vector<double> e2e_graph_key;
 };
 
 namespace monte {
@@ -225,14 +271,29 @@ hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* 
 hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, unsigned int* queue,
                                McCounters* counters, int batch, int block, int grid, hipStream_t s);
 hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs, hipStream_t s);
+// First-flight label sets: per brick of b^3 voxels widened by one voxel
+// (bricks [nb2][nb1][nb0]), then per view of [view_begin, view_begin +
+// n_views) and pixel in aim order (sets [n_views][nu nv]); cs64 holds FP64
+// (cos, sin) of every view of the scan.
+hipError_t launch_brick_sets(const uint8_t* labels, int nx, int ny, int nz, int b, const int nb[3], uint8_t* bricks,
+                             hipStream_t s);
+hipError_t launch_ray_sets(const RayGrid& R, const double* cs64, int view_begin, int n_views, const uint8_t* bricks,
+                           uint8_t* sets, hipStream_t s);
 // Bounding box (voxel indices, inclusive) of voxels whose clamped label has bit set in `solid`:
 // box[0..2] = minima, box[3..5] = maxima; the caller initialises it to {INT_MAX x 3, -1 x 3}.
 hipError_t launch_support_box(const uint8_t* labels, int nx, int ny, int nz, uint32_t solid, int* box, hipStream_t s);
 hipError_t launch_primary_exact(const PrimaryGeom& P, const double* cs64, const uint8_t* labels,
                                 const McTables* tables, float e_kev, const float* spec_w, int use_spec, float* out,
                                 hipStream_t s);
-hipError_t launch_counts_to_lineint(const double* counts, float* out, int n_views, int nu, int nv,
-                                    uint64_t photons_per_view, hipStream_t s);
+// counts2 (nullable) is added to counts before the conversion
+hipError_t launch_counts_to_lineint(const double* counts, const double* counts2, float* out, int n_views, int nu,
+                                    int nv, uint64_t photons_per_view, hipStream_t s);
+// out = a - b; partial[blk] = FP64 sum of out^2 over block blk's fixed share
+// (grid-stride, reduced in a fixed order), then loss = 0.5 * sum(partial) in
+// one more fixed-order pass: deterministic, so graph replays are bitwise equal.
+constexpr int kLossBlocks = 1024;
+hipError_t launch_residual_loss(const float* a, const float* b, float* out, size_t n, double* partial, double* loss,
+                                hipStream_t s);
 hipError_t launch_counts_to_lineint_flat(const double* counts, const double* flat, int flat_views, float* out,
                                          int n_views, int nu, int nv, hipStream_t s);
 

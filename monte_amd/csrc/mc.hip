@@ -249,17 +249,31 @@ __device__ inline int tally(const Photon& p, const McJob& J) {
   return kTallyScatter;
 }
 
+// Label set meaning "any label": the flight takes the global majorant.
+constexpr uint32_t kAnyLabel = 16u;
+
 // Starts a flight from the current position; returns false if the photon
-// leaves without entering the phantom box (caller tallies it).
-__device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables& wt) {
+// leaves without entering the phantom box, or meets nothing that attenuates
+// (caller tallies it).  The Woodcock majorant is the largest mu among the
+// labels of `set` (bit l: label l in 0..3; kAnyLabel: every label).
+__device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables& wt, uint32_t set) {
   float t0, t1;
   if (!ray_box(p.px, p.py, p.pz, p.dx, p.dy, p.dz, J, t0, t1) || !(t1 > 0.f)) return false;
-  const float mm = wt.mumax[p.eb];
-  if (!(mm > 0.f)) return false;
   p.mu0 = wt.mu[0][p.eb];
   p.mu1 = wt.mu[1][p.eb];
   p.mu2 = wt.mu[2][p.eb];
   p.mu3 = wt.mu[3][p.eb];
+  float mm;
+  if (set & kAnyLabel) {
+    mm = wt.mumax[p.eb];
+  } else {
+    mm = 0.f;
+    if (set & 1u) mm = fmaxf(mm, p.mu0);
+    if (set & 2u) mm = fmaxf(mm, p.mu1);
+    if (set & 4u) mm = fmaxf(mm, p.mu2);
+    if (set & 8u) mm = fmaxf(mm, p.mu3);
+  }
+  if (!(mm > 0.f)) return false;
   if (t0 > 0.f) {
     p.px = fmaf(t0, p.dx, p.px); p.py = fmaf(t0, p.dy, p.py); p.pz = fmaf(t0, p.dz, p.pz);
     t1 -= t0;
@@ -449,7 +463,11 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           p.e = e;
           p.eb = ebin_of(e);
           p.nint = 0;
-          if (start_flight(p, *Jp, wt)) {
+          // the first flight's majorant comes from the labels its line crosses
+          const uint32_t set = Jp->ray_set
+                                   ? (uint32_t)Jp->ray_set[(size_t)(unit_view - Jp->view_begin) * npix + pix]
+                                   : kAnyLabel;
+          if (start_flight(p, *Jp, wt, set)) {
             state = kFly;
           } else {
             MONTE_NOTE_TALLY(tally(p, *Jp));
@@ -493,7 +511,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             rotate_dir(p.dx, p.dy, p.dz, cth, phi);
           }
           p.nint += 1;
-          if (start_flight(p, *Jp, wt)) {
+          if (start_flight(p, *Jp, wt, kAnyLabel)) {
             state = kFly;
           } else {
             MONTE_NOTE_TALLY(tally(p, *Jp));
@@ -545,6 +563,133 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   }
   if (job >= 0) flush(counters + job);
 #undef MONTE_NOTE_TALLY
+}
+
+// --------------------------------------------- first-flight label sets
+// IEEE FP64 add, subtract and multiply that are never fused into FMAs, so the
+// label sets follow the oracle's operations bit for bit.
+__device__ inline double xadd(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ inline double xsub(double a, double b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+__device__ inline double xmul(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
+__device__ inline uint32_t label_bit(int lab) { return lab < 4 ? 1u << lab : kAnyLabel; }
+
+// One thread per brick: the labels of the brick's voxels and of their
+// neighbours one voxel outside it, so a sample point rounded across a brick
+// face into a voxel of the next brick is still covered.
+__global__ __launch_bounds__(256) void brick_sets_kernel(const uint8_t* __restrict__ labels, int nx, int ny, int nz,
+                                                         int b, int nb0, int nb1, int nb2,
+                                                         uint8_t* __restrict__ bricks) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nb0 * nb1 * nb2) return;
+  const int bx = (int)(i % nb0), by = (int)((i / nb0) % nb1), bz = (int)(i / ((long)nb0 * nb1));
+  const int x0 = max(bx * b - 1, 0), x1 = min(bx * b + b, nx - 1);
+  const int y0 = max(by * b - 1, 0), y1 = min(by * b + b, ny - 1);
+  const int z0 = max(bz * b - 1, 0), z1 = min(bz * b + b, nz - 1);
+  uint32_t m = 0;
+  for (int z = z0; z <= z1; ++z)
+    for (int y = y0; y <= y1; ++y) {
+      const uint8_t* row = labels + ((size_t)z * ny + y) * nx;
+      for (int x = x0; x <= x1; ++x) m |= label_bit(row[x]);
+    }
+  bricks[i] = (uint8_t)m;
+}
+
+// One thread per (view, pixel): the union of the brick sets along the segment
+// source -> pixel centre inside the flight box, by index-stepping traversal of
+// the brick grid (every crossing recomputed from its plane index), or
+// kAnyLabel if the segment misses the box.
+__global__ __launch_bounds__(256) void ray_sets_kernel(RayGrid R, const double* __restrict__ cs64, int view_begin,
+                                                       int n_views, const uint8_t* __restrict__ bricks,
+                                                       uint8_t* __restrict__ sets) {
+  const long npix = (long)R.nu * R.nv;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= npix * n_views) return;
+  const int kk = (int)(i / npix);
+  const int pix = (int)(i - (long)kk * npix);
+  const int iv = pix / R.nu, iu = pix - iv * R.nu;
+  const double cb = cs64[2 * (view_begin + kk)], sb = cs64[2 * (view_begin + kk) + 1];
+  const double yp = xmul(xsub(xadd((double)iu, 0.5), R.un0), R.u_step);
+  const double zp = xmul(xsub(xadd((double)iv, 0.5), R.vn0), R.v_step);
+  const double S[3] = {xmul(-R.sod, cb), xmul(-R.sod, sb), 0.0};
+  const double P[3] = {xsub(xmul(R.xd, cb), xmul(yp, sb)), xadd(xmul(R.xd, sb), xmul(yp, cb)), zp};
+  double d[3], inv[3];
+  double lo = 0.0, hi = 1.0;
+  uint32_t m = 0;
+  bool miss = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    d[a] = xsub(P[a], S[a]);
+    if (d[a] != 0.0) {
+      inv[a] = 1.0 / d[a];
+      const double ta = xmul(xsub(R.c0[a], S[a]), inv[a]), tb = xmul(xsub(R.c1[a], S[a]), inv[a]);
+      const double tl = ta < tb ? ta : tb, th = ta < tb ? tb : ta;
+      if (tl > lo) lo = tl;
+      if (th < hi) hi = th;
+    } else {
+      inv[a] = 0.0;
+      if (S[a] < R.c0[a] || S[a] >= R.c1[a]) miss = true;
+    }
+  }
+  if (miss || !(hi > lo)) {
+    sets[i] = (uint8_t)kAnyLabel;
+    return;
+  }
+  const double span = xsub(hi, lo);
+  const double half = xmul(span, 0.5);
+  const double tm = xadd(lo, half < 1e-9 ? half : 1e-9);  // a point just inside
+  int idx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double q = xadd(S[a], xmul(tm, d[a]));
+    const int j = (int)floor(xsub(q, R.org[a]) / R.bw[a]);
+    idx[a] = j < 0 ? 0 : (j > R.nb[a] - 1 ? R.nb[a] - 1 : j);
+  }
+  for (int guard = 0; guard < 4 * (R.nb[0] + R.nb[1] + R.nb[2]) + 8; ++guard) {
+    m |= bricks[((size_t)idx[2] * R.nb[1] + idx[1]) * R.nb[0] + idx[0]];
+    double tn[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (d[a] > 0.0) tn[a] = xmul(xsub(xadd(R.org[a], xmul((double)(idx[a] + 1), R.bw[a])), S[a]), inv[a]);
+      else if (d[a] < 0.0) tn[a] = xmul(xsub(xadd(R.org[a], xmul((double)idx[a], R.bw[a])), S[a]), inv[a]);
+      else tn[a] = INFINITY;
+    }
+    int a = 0;
+    if (tn[1] < tn[a]) a = 1;
+    if (tn[2] < tn[a]) a = 2;
+    if (!(tn[a] < hi)) break;
+    idx[a] += d[a] > 0.0 ? 1 : -1;
+    if (idx[a] < 0 || idx[a] >= R.nb[a]) break;
+  }
+  sets[i] = (uint8_t)m;
+}
+
+hipError_t launch_brick_sets(const uint8_t* labels, int nx, int ny, int nz, int b, const int nb[3], uint8_t* bricks,
+                             hipStream_t s) {
+  const long n = (long)nb[0] * nb[1] * nb[2];
+  if (n <= 0 || b < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(brick_sets_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, labels, nx, ny, nz, b,
+                     nb[0], nb[1], nb[2], bricks);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_sets(const RayGrid& R, const double* cs64, int view_begin, int n_views, const uint8_t* bricks,
+                           uint8_t* sets, hipStream_t s) {
+  const long n = (long)R.nu * R.nv * n_views;
+  if (n <= 0) return hipSuccess;
+  if ((n + 255) / 256 > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ray_sets_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, R, cs64, view_begin,
+                     n_views, bricks, sets);
+  return hipGetLastError();
 }
 
 __global__ void mc_reset_kernel(unsigned int* queue, McCounters* counters, int n_jobs) {
@@ -743,8 +888,11 @@ hipError_t launch_primary_exact(const PrimaryGeom& P, const double* cs64, const 
 // CBCT_real325.cu:267-294: clamp N to [1, N0], L = ln N0 - ln N, with N0 the
 // photons aimed at the pixel: floor(N/npix) + (a < N mod npix), a = iv nu + iu
 // the pixel's place in the aim order.
-__global__ void counts_to_lineint_kernel(const double* __restrict__ counts, float* __restrict__ out, int n_views,
-                                         int nu, int nv, uint64_t photons_per_view) {
+// `counts2`, when given, is added to `counts` first (primary + scatter tallies
+// in one pass).
+__global__ void counts_to_lineint_kernel(const double* __restrict__ counts, const double* __restrict__ counts2,
+                                         float* __restrict__ out, int n_views, int nu, int nv,
+                                         uint64_t photons_per_view) {
   const int npix = nu * nv;
   const size_t n = (size_t)n_views * npix;
   const uint64_t base = photons_per_view / (uint64_t)npix, rem = photons_per_view % (uint64_t)npix;
@@ -753,7 +901,7 @@ __global__ void counts_to_lineint_kernel(const double* __restrict__ counts, floa
     const int iu = pix / nv, iv = pix - iu * nv;
     const int a = iv * nu + iu;
     const double n0 = (double)(base + ((uint64_t)a < rem ? 1 : 0));
-    double c = counts[i];
+    double c = counts2 ? counts[i] + counts2[i] : counts[i];
     c = c > n0 ? n0 : c;
     c = c < 1.0 ? 1.0 : c;
     out[i] = n0 > 0.0 ? (float)(log(n0) - log(c)) : 0.f;
@@ -784,14 +932,14 @@ hipError_t launch_counts_to_lineint_flat(const double* counts, const double* fla
   return hipGetLastError();
 }
 
-hipError_t launch_counts_to_lineint(const double* counts, float* out, int n_views, int nu, int nv,
-                                    uint64_t photons_per_view, hipStream_t s) {
+hipError_t launch_counts_to_lineint(const double* counts, const double* counts2, float* out, int n_views, int nu,
+                                    int nv, uint64_t photons_per_view, hipStream_t s) {
   const size_t n = (size_t)n_views * nu * nv;
   size_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(counts_to_lineint_kernel, dim3((unsigned)blocks), dim3(256), 0, s, counts, out, n_views, nu,
-                     nv, photons_per_view);
+  hipLaunchKernelGGL(counts_to_lineint_kernel, dim3((unsigned)blocks), dim3(256), 0, s, counts, counts2, out, n_views,
+                     nu, nv, photons_per_view);
   return hipGetLastError();
 }
 

@@ -149,6 +149,23 @@ def mc_batch(jobs: list, ctx: Context | None = None, device: int = 0):
     return [s.as_dict() for s in stats]
 
 
+def first_flight_sets(ph: LabelPhantom, geom: Geometry, brick: int, labels_dev: torch.Tensor | None = None,
+                      ctx: Context | None = None, device: int = 0) -> torch.Tensor:
+    """The label sets bounding each first flight's Woodcock majorant, uint8
+    [views][nv * nu] (pixel iv * nu + iu): bit l for label l < 4, 16 for any
+    label >= 4 (or a line missing the flight box)."""
+    ctx = ctx or default_context(device)
+    dev = torch.device("cuda", ctx.device)
+    if labels_dev is None:
+        labels_dev = device_labels(ph, dev)
+    _check_labels(labels_dev, ph, ctx)
+    out = torch.zeros((geom.n_views, geom.nu * geom.nv), dtype=torch.uint8, device=dev)
+    g = geom.c()
+    p = ph.c(labels_dev.data_ptr())
+    ctx.call("monte_mc_first_flight_sets", C.byref(g), C.byref(p), int(brick), C.c_void_p(out.data_ptr()))
+    return out
+
+
 def primary_exact(ph: LabelPhantom, geom: Geometry, e_kev: float = 60.0, spectrum: np.ndarray | None = None,
                   labels_dev: torch.Tensor | None = None, ctx: Context | None = None, device: int = 0):
     ctx = ctx or default_context(device)

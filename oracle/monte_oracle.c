@@ -660,6 +660,115 @@ int orc_spectrum_bin(const double *cdf, double u) {
   return e;
 }
 
+/* ------------------------------------------------ first-flight majorant
+ * A photon's first flight runs along the line from the source to the centre
+ * of its pixel, so the labels that line crosses bound the attenuation it can
+ * meet: the Woodcock majorant of that flight may be the largest mu among them
+ * instead of among all labels (Woodcock tracking stays exact for any majorant
+ * at least mu everywhere along the flight).  Label sets are bit masks: bit l
+ * for labels 0..3, bit 4 for any label 4..255.
+ *
+ * orc_brick_masks: the label set of every brick of b^3 voxels, taken over the
+ * brick widened by one voxel on every side, so a point whose voxel is at most
+ * one voxel away from a brick (a sample point rounded across a brick face) is
+ * covered by that brick's set.
+ * orc_ray_masks: for view k and pixel pix (aim order iv * nu + iu), the union
+ * of the sets of the bricks the segment source -> pixel centre crosses inside
+ * the box [c0, c1] (an index-stepping traversal of the brick grid), or 16
+ * (every label) when the segment misses the box.  The arithmetic is plain
+ * IEEE FP64 without fused operations, in a fixed order, so the product
+ * kernel, written with the same operations, yields the same masks bit for bit. */
+static int label_bit(int lab) { return lab < 4 ? 1 << lab : 16; }
+
+void orc_brick_masks(const orc_phantom *ph, int b, uint8_t *out) {
+  const int nbx = (ph->nx + b - 1) / b, nby = (ph->ny + b - 1) / b, nbz = (ph->nz + b - 1) / b;
+  for (int bz = 0; bz < nbz; ++bz)
+    for (int by = 0; by < nby; ++by)
+      for (int bx = 0; bx < nbx; ++bx) {
+        int m = 0;
+        const int z0 = bz * b - 1 < 0 ? 0 : bz * b - 1, z1 = bz * b + b < ph->nz ? bz * b + b : ph->nz - 1;
+        const int y0 = by * b - 1 < 0 ? 0 : by * b - 1, y1 = by * b + b < ph->ny ? by * b + b : ph->ny - 1;
+        const int x0 = bx * b - 1 < 0 ? 0 : bx * b - 1, x1 = bx * b + b < ph->nx ? bx * b + b : ph->nx - 1;
+        for (int z = z0; z <= z1; ++z)
+          for (int y = y0; y <= y1; ++y)
+            for (int x = x0; x <= x1; ++x) m |= label_bit(ph->labels[((size_t)z * ph->ny + y) * ph->nx + x]);
+        out[((size_t)bz * nby + by) * nbx + bx] = (uint8_t)m;
+      }
+}
+
+/* label set of the segment S -> P (see above); org: grid corner, bw: brick size */
+static int ray_set(const double S[3], const double P[3], const double c0[3], const double c1[3],
+                   const double org[3], const double bw[3], const int nb[3], const uint8_t *bricks) {
+  double d[3], inv[3];
+  double lo = 0.0, hi = 1.0;
+  for (int a = 0; a < 3; ++a) {
+    d[a] = P[a] - S[a];
+    if (d[a] != 0.0) {
+      inv[a] = 1.0 / d[a];
+      const double ta = (c0[a] - S[a]) * inv[a], tb = (c1[a] - S[a]) * inv[a];
+      const double tl = ta < tb ? ta : tb, th = ta < tb ? tb : ta;
+      if (tl > lo) lo = tl;
+      if (th < hi) hi = th;
+    } else {
+      inv[a] = 0.0;
+      if (S[a] < c0[a] || S[a] >= c1[a]) return 16;
+    }
+  }
+  if (!(hi > lo)) return 16;
+  const double span = hi - lo;
+  const double tm = lo + (span * 0.5 < 1e-9 ? span * 0.5 : 1e-9); /* a point just inside */
+  int idx[3];
+  for (int a = 0; a < 3; ++a) {
+    const double q = S[a] + tm * d[a];
+    int i = (int)floor((q - org[a]) / bw[a]);
+    idx[a] = i < 0 ? 0 : (i > nb[a] - 1 ? nb[a] - 1 : i);
+  }
+  int m = 0;
+  for (;;) {
+    m |= bricks[((size_t)idx[2] * nb[1] + idx[1]) * nb[0] + idx[0]];
+    double tn[3];
+    for (int a = 0; a < 3; ++a) {
+      if (d[a] > 0.0) tn[a] = (org[a] + (double)(idx[a] + 1) * bw[a] - S[a]) * inv[a];
+      else if (d[a] < 0.0) tn[a] = (org[a] + (double)idx[a] * bw[a] - S[a]) * inv[a];
+      else tn[a] = INFINITY;
+    }
+    int a = 0;
+    if (tn[1] < tn[a]) a = 1;
+    if (tn[2] < tn[a]) a = 2;
+    if (!(tn[a] < hi)) break;
+    idx[a] += d[a] > 0.0 ? 1 : -1;
+    if (idx[a] < 0 || idx[a] >= nb[a]) break;
+  }
+  return m;
+}
+
+void orc_ray_masks(const orc_geom *g, const orc_phantom *ph, const double c0[3], const double c1[3],
+                   const uint8_t *bricks, int b, int view_begin, int n_views, uint8_t *out) {
+  double un0, u_step, vn0, v_step;
+  if (g->preset == ORC_PRESET_BP3D20) {
+    un0 = 0.5 * g->nu; u_step = -g->du; vn0 = 0.5 * g->nv; v_step = -g->dv;
+  } else {
+    un0 = 0.5 * g->nu - g->u_off / g->du; u_step = g->du;
+    vn0 = 0.5 * g->nv - g->v_off / g->dv; v_step = g->dv;
+  }
+  double org[3], b1[3];
+  ph_bounds(ph, org, b1);
+  const double bw[3] = {b * ph->vx, b * ph->vy, b * ph->vz};
+  const int nb[3] = {(ph->nx + b - 1) / b, (ph->ny + b - 1) / b, (ph->nz + b - 1) / b};
+  const size_t npix = (size_t)g->nu * g->nv;
+  const double xd = g->sdd - g->sod;
+  for (int kk = 0; kk < n_views; ++kk) {
+    const double beta = view_angle(g, view_begin + kk), cb = cos(beta), sb = sin(beta);
+    const double S[3] = {-g->sod * cb, -g->sod * sb, 0.0};
+    for (size_t pix = 0; pix < npix; ++pix) {
+      const int iv = (int)(pix / g->nu), iu = (int)(pix % g->nu);
+      const double yp = ((double)iu + 0.5 - un0) * u_step, zp = ((double)iv + 0.5 - vn0) * v_step;
+      const double P[3] = {xd * cb - yp * sb, xd * sb + yp * cb, zp};
+      out[(size_t)kk * npix + pix] = (uint8_t)ray_set(S, P, c0, c1, org, bw, nb, bricks);
+    }
+  }
+}
+
 typedef struct {
   float mu[ORC_MAX_MAT][ORC_NROWS];  /* linear attenuation per label (cm^-1), FP32-rounded */
   float fa[ORC_MAX_MAT][ORC_NROWS];  /* photo / total */
@@ -704,6 +813,7 @@ typedef struct {
   pthread_mutex_t mu;
   long chunks_per_view;
   double c0[3], c1[3]; /* flight box: bounding box of the attenuating voxels */
+  const uint8_t *ray_sets; /* [views][npix] first-flight label sets, or NULL (global majorant) */
 } orc_mc_ctx;
 
 /* Bounding box of the voxels whose label attenuates at some energy (labels
@@ -781,19 +891,32 @@ static void mc_items(void *vctx, long i0, long i1) {
       if (cfg->spectrum_cdf) E = orc_spectrum_bin(cfg->spectrum_cdf, orc_next(&rs));
       int eb = ebin_of(E);
       int nint = 0;
+      int first = 1;
+      const int ray_set = c->ray_sets ? c->ray_sets[(size_t)(k - cfg->view_begin) * npix + pix] : 16;
       for (;;) {
         /* flight: Woodcock inside the box of attenuating voxels, straight
          * line outside */
         double t0, t1;
         int real = 0;
-        if (ray_box(pos, dir, c->c0, c->c1, &t0, &t1) && t1 > 0 && mt->mumax[eb] > 0.f) {
+        /* majorant: the largest mu among the labels of the first flight's line,
+         * else among all labels */
+        const int set = first ? ray_set : 16;
+        float mm = 0.f;
+        if (set & 16) {
+          mm = mt->mumax[eb];
+        } else {
+          for (int l = 0; l < 4; ++l)
+            if ((set >> l) & 1 && mt->mu[l][eb] > mm) mm = mt->mu[l][eb];
+        }
+        first = 0;
+        if (ray_box(pos, dir, c->c0, c->c1, &t0, &t1) && t1 > 0 && mm > 0.f) {
           /* advance to the box entry so flight parameters stay small */
           if (t0 > 0) {
             pos[0] += t0 * dir[0]; pos[1] += t0 * dir[1]; pos[2] += t0 * dir[2];
             t1 -= t0;
           }
           double t = 0;
-          const double mumax = mt->mumax[eb];
+          const double mumax = mm;
           /* two Woodcock steps per block: (distance, accept, distance, accept) */
           for (int ended = 0; !ended;) {
             double u[4];
@@ -888,6 +1011,17 @@ void orc_mc_run(const orc_geom *g, const orc_phantom *ph, const orc_tables *t, c
   orc_mc_ctx c;
   c.g = g; c.ph = ph; c.cfg = cfg; c.mt = mt;
   support_box(ph, mt, c.c0, c.c1);
+  uint8_t *bricks = NULL, *sets = NULL;
+  c.ray_sets = NULL;
+  if (cfg->ray_brick > 0 && cfg->view_end > cfg->view_begin) {
+    const int b = cfg->ray_brick;
+    const size_t nbr = (size_t)((ph->nx + b - 1) / b) * ((ph->ny + b - 1) / b) * ((ph->nz + b - 1) / b);
+    bricks = (uint8_t *)malloc(nbr);
+    sets = (uint8_t *)malloc((size_t)(cfg->view_end - cfg->view_begin) * g->nu * g->nv);
+    orc_brick_masks(ph, b, bricks);
+    orc_ray_masks(g, ph, c.c0, c.c1, bricks, b, cfg->view_begin, cfg->view_end - cfg->view_begin, sets);
+    c.ray_sets = sets;
+  }
   c.primary = primary; c.scatter = scatter; c.stats = stats ? stats : &local;
   pthread_mutex_init(&c.mu, NULL);
   int nv = cfg->view_end - cfg->view_begin;
@@ -898,5 +1032,14 @@ void orc_mc_run(const orc_geom *g, const orc_phantom *ph, const orc_tables *t, c
   c.chunks_per_view = cpv;
   orc_parallel_for((long)nv * cpv, 1, nthreads, mc_items, &c);
   pthread_mutex_destroy(&c.mu);
+  free(bricks);
+  free(sets);
+  free(mt);
+}
+
+void orc_flight_box(const orc_phantom *ph, const orc_tables *t, double c0[3], double c1[3]) {
+  orc_mc_tables *mt = (orc_mc_tables *)malloc(sizeof(orc_mc_tables));
+  build_mc_tables(ph, t, mt);
+  support_box(ph, mt, c0, c1);
   free(mt);
 }

@@ -135,12 +135,26 @@ typedef struct orc_mc_cfg {
   uint32_t seed_lo, seed_hi;
   uint32_t job;
   int view_begin, view_end; /* subset of views to simulate */
+  int ray_brick;            /* > 0: first flights take the majorant of the labels their line crosses,
+                               from bricks of ray_brick^3 voxels (orc_ray_masks); 0: global majorant */
 } orc_mc_cfg;
 
 typedef struct orc_mc_stats {
   uint64_t histories, steps, interactions, photo, coherent, compton;
   uint64_t tallied_primary, tallied_scatter, missed, capped;
 } orc_mc_stats;
+
+/* Label sets (bit l: label l in 0..3, bit 4: any label >= 4) of the first
+ * flight's line: bricks of b^3 voxels widened by one voxel
+ * (out [ceil(nz/b)][ceil(ny/b)][ceil(nx/b)]), and per view of [view_begin,
+ * view_begin + n_views) and pixel in aim order the union over the bricks the
+ * source -> pixel-centre segment crosses inside the box [c0, c1]
+ * (out [n_views][nv * nu]). */
+void orc_brick_masks(const orc_phantom *ph, int b, uint8_t *out);
+void orc_ray_masks(const orc_geom *g, const orc_phantom *ph, const double c0[3], const double c1[3],
+                   const uint8_t *bricks, int b, int view_begin, int n_views, uint8_t *out);
+/* the flight box of the transport: bounding box of the attenuating voxels */
+void orc_flight_box(const orc_phantom *ph, const orc_tables *t, double c0[3], double c1[3]);
 
 /* primary, scatter: [view][nu][nv] FP64 tallies (accumulated) */
 void orc_mc_run(const orc_geom *g, const orc_phantom *ph, const orc_tables *t,

@@ -54,7 +54,7 @@ class McCfg(C.Structure):
         ("photons_per_view", C.c_uint64), ("e_kev", C.c_double), ("spectrum_cdf", C.c_void_p),
         ("max_interactions", C.c_int), ("coherent_mode", C.c_int),
         ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("job", C.c_uint32),
-        ("view_begin", C.c_int), ("view_end", C.c_int),
+        ("view_begin", C.c_int), ("view_end", C.c_int), ("ray_brick", C.c_int),
     ]
 
 
@@ -102,6 +102,9 @@ def lib():
         L.orc_thomson_mu.restype = C.c_double
         L.orc_thomson_mu.argtypes = [C.c_double]
         L.orc_rotate_dir.argtypes = [P, C.c_double, C.c_double]
+        L.orc_brick_masks.argtypes = [C.POINTER(Phantom), C.c_int, P]
+        L.orc_ray_masks.argtypes = [C.POINTER(Geom), C.POINTER(Phantom), P, P, P, C.c_int, C.c_int, C.c_int, P]
+        L.orc_flight_box.argtypes = [C.POINTER(Phantom), C.POINTER(Tables), P, P]
         L.orc_spectrum_bin.restype = C.c_int
         L.orc_spectrum_bin.argtypes = [P, C.c_double]
         _lib = L
@@ -286,7 +289,7 @@ def siddon_lengths(ph, p0, p1, n_labels=256):
 
 
 def mc_run(g, ph, tb, photons_per_view, e_kev=60.0, spectrum_cdf=None, max_interactions=5,
-           coherent_mode=COH_FORWARD, seed=20261202, job=0, views=None, nthreads=None):
+           coherent_mode=COH_FORWARD, seed=20261202, job=0, views=None, nthreads=None, ray_brick=0):
     cfg = McCfg()
     cfg.photons_per_view = int(photons_per_view)
     cfg.e_kev = float(e_kev)
@@ -301,6 +304,7 @@ def mc_run(g, ph, tb, photons_per_view, e_kev=60.0, spectrum_cdf=None, max_inter
     cfg.job = job
     v0, v1 = (0, g.n_views) if views is None else views
     cfg.view_begin, cfg.view_end = v0, v1
+    cfg.ray_brick = int(ray_brick)
     prim = np.zeros((g.n_views, g.nu, g.nv), dtype=np.float64)
     scat = np.zeros_like(prim)
     st = McStats()
@@ -330,3 +334,29 @@ def rotate_dir(d, cos_theta, phi):
 def spectrum_bin(cdf, u):
     c = np.ascontiguousarray(cdf, dtype=np.float64)
     return lib().orc_spectrum_bin(_p(c), float(u))
+
+
+def brick_masks(ph, b):
+    """Label set (bit l: label l < 4, bit 4: label >= 4) of every brick of b^3
+    voxels widened by one voxel: [ceil(nz/b)][ceil(ny/b)][ceil(nx/b)] uint8."""
+    out = np.zeros((-(-ph.nz // b), -(-ph.ny // b), -(-ph.nx // b)), dtype=np.uint8)
+    lib().orc_brick_masks(C.byref(ph), int(b), _p(out))
+    return out
+
+
+def flight_box(ph, tb):
+    c0 = np.zeros(3)
+    c1 = np.zeros(3)
+    lib().orc_flight_box(C.byref(ph), C.byref(tb), _p(c0), _p(c1))
+    return c0, c1
+
+
+def ray_masks(g, ph, tb, b, views=None):
+    """First-flight label sets per view and pixel (aim order iv * nu + iu):
+    [n_views][nv * nu] uint8."""
+    v0, v1 = (0, g.n_views) if views is None else views
+    c0, c1 = flight_box(ph, tb)
+    bricks = brick_masks(ph, b)
+    out = np.zeros((v1 - v0, g.nu * g.nv), dtype=np.uint8)
+    lib().orc_ray_masks(C.byref(g), C.byref(ph), _p(c0), _p(c1), _p(bricks), int(b), v0, v1 - v0, _p(out))
+    return out

@@ -42,6 +42,7 @@ def test_library_exports_every_declared_symbol():
 STRUCTS = {
     "monte_geom": "Geom", "monte_phantom": "Phantom", "monte_mc_cfg": "McCfg", "monte_mc_stats": "McStats",
     "monte_mc_job": "McJob", "monte_tuning": "Tuning", "monte_primitive": "Primitive",
+    "monte_e2e_buffers": "E2EBuffers",
 }
 
 

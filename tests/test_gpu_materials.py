@@ -90,7 +90,8 @@ def test_mc_four_labels_matches_oracle(ctx3, oracle):
     p_ref, s_ref, st_ref = oracle.mc_run(oracle_geom(oracle, geom), oracle_phantom(oracle, ph),
                                          oracle.tables(_tables3(lambda p: oracle.read_xcom_csv(p)[0])), N,
                                          e_kev=s.e_kev, spectrum_cdf=None, max_interactions=s.max_interactions,
-                                         coherent_mode=s.coherent, seed=s.seed, job=s.job)
+                                         coherent_mode=s.coherent, seed=s.seed, job=s.job,
+                                         ray_brick=ctx3.tuning["mc_ray_brick"])
     print("gpu", st)
     print("cpu", st_ref)
     assert st["histories"] == st_ref["histories"] == N * geom.n_views

@@ -53,7 +53,7 @@ def _mc_pair(ctx, oracle, geom, ph, photons, **kw):
     p_ref, s_ref, st_ref = oracle.mc_run(oracle_geom(oracle, geom), oracle_phantom(oracle, ph), _tables(oracle),
                                          photons, e_kev=s.e_kev, spectrum_cdf=spec_cdf,
                                          max_interactions=s.max_interactions, coherent_mode=s.coherent,
-                                         seed=s.seed, job=s.job)
+                                         seed=s.seed, job=s.job, ray_brick=ctx.tuning["mc_ray_brick"])
     return prim.cpu().numpy(), scat.cpu().numpy(), st, p_ref, s_ref, st_ref
 
 
@@ -218,9 +218,11 @@ def test_primary_exact_box_closed_form(ctx):
 
 
 def test_woodcock_majorant_invariance(ctx):
-    """A label that no voxel carries, made dense, raises the Woodcock majorant (more
-    virtual steps per history) without changing the physics: both runs sample the
-    same distributions, so their tallies agree pixel by pixel within Poisson noise."""
+    """A label that no voxel carries, made dense, raises the global Woodcock majorant
+    (more virtual steps per history) without changing the physics: both runs
+    sample the same distributions, so their tallies agree pixel by pixel within
+    Poisson noise.  (Run with the global majorant for every flight: the
+    first-flight label sets would leave the unused label out.)"""
     import torch
 
     from monte_amd import geometry as G
@@ -233,8 +235,13 @@ def test_woodcock_majorant_invariance(ctx):
                                   base.origin)
     assert 3 not in np.unique(base.labels)
     s = McSettings(photons_per_view=4_000_000, e_kev=60.0)
-    pa, sa, sta = mc_project(base, geom, s, ctx=ctx)
-    pb, sb, stb = mc_project(heavy, geom, s, ctx=ctx)
+    brick = ctx.tuning["mc_ray_brick"]
+    ctx.set_tuning(mc_ray_brick=0)
+    try:
+        pa, sa, sta = mc_project(base, geom, s, ctx=ctx)
+        pb, sb, stb = mc_project(heavy, geom, s, ctx=ctx)
+    finally:
+        ctx.set_tuning(mc_ray_brick=brick)
     print("steps per history", sta["steps"] / sta["histories"], "->", stb["steps"] / stb["histories"])
     assert stb["steps"] > 1.5 * sta["steps"]
     print("tallied scatter", sta["tallied_scatter"], stb["tallied_scatter"])

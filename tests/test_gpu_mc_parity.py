@@ -57,7 +57,8 @@ def _run_pair(ctx, oracle, geom, ph, settings):
     p_ref, s_ref, st_ref = oracle.mc_run(oracle_geom(oracle, geom), oracle_phantom(oracle, ph), _tables(oracle),
                                          settings.photons_per_view, e_kev=settings.e_kev, spectrum_cdf=cdf,
                                          max_interactions=settings.max_interactions,
-                                         coherent_mode=settings.coherent, seed=settings.seed, job=settings.job)
+                                         coherent_mode=settings.coherent, seed=settings.seed, job=settings.job,
+                                         ray_brick=ctx.tuning["mc_ray_brick"])
     return (prim.cpu().numpy(), scat.cpu().numpy()), st, (p_ref, s_ref), st_ref
 
 
