@@ -201,6 +201,19 @@ int monte_fdk_adjoint(monte_ctx *ctx, const monte_geom *g, const float *dvol, fl
  * counts per pixel (accumulated into primary/scatter [views][nu][nv]). */
 int monte_mc_project(monte_ctx *ctx, const monte_geom *g, const monte_phantom *ph, const monte_mc_cfg *cfg,
                      double *primary, double *scatter, monte_mc_stats *stats);
+/* monte_mc_project without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate during a call (graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Device
+ * pointers only.  The launch's descriptors, attenuation tables, spectrum,
+ * per-view table, work queue and counters live in a static 128 KiB device
+ * array per GPU, which bounds the geometry to about 9500 views
+ * (MONTE_ERR_INVALID beyond); every call on a GPU reuses that GPU's array, so
+ * calls that overlap on one GPU race.  Every flight takes the global Woodcock
+ * majorant (the first-flight label sets of tuning mc_ray_brick need memory in
+ * proportion to views x pixels), so results equal monte_mc_project's run with
+ * mc_ray_brick = 0. */
+int monte_mc_project_noalloc(monte_ctx *ctx, const monte_geom *g, const monte_phantom *ph, const monte_mc_cfg *cfg,
+                             double *primary, double *scatter, monte_mc_stats *stats);
 /* The label sets that bound the first flights' Woodcock majorants (tuning
  * mc_ray_brick), for inspection and tests; no reference counterpart.  sets:
  * device uint8 [n_views][nv * nu], pixel iv * nu + iu (the aim order); bit l

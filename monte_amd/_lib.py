@@ -119,6 +119,8 @@ SIGNATURES = {
     "monte_fdk_adjoint": (C.c_int, [_CTX, C.POINTER(Geom), _P, _P, C.c_int]),
     "monte_mc_project": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.POINTER(McCfg), _P, _P,
                                    C.POINTER(McStats)]),
+    "monte_mc_project_noalloc": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.POINTER(McCfg), _P, _P,
+                                           C.POINTER(McStats)]),
     "monte_mc_first_flight_sets": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.c_int, _P]),
     "monte_mc_batch": (C.c_int, [_CTX, C.POINTER(McJob), C.c_int, C.POINTER(McStats)]),
     "monte_primary_exact": (C.c_int, [_CTX, C.POINTER(Geom), C.POINTER(Phantom), C.c_double, _P, _P]),

@@ -688,6 +688,14 @@ int monte_create(monte_ctx** out, int device) {
     delete c;
     return MONTE_ERR_HIP;
   }
+  // the allocation-free entry points' scratch: located once per device
+  if (noalloc_setup(device) != 0) {
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return MONTE_ERR_HIP;
+  }
   int cu = 0;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->n_cu = cu;
   c->tuning = default_tuning();
@@ -966,6 +974,31 @@ int monte_mc_project(monte_ctx* c, const monte_geom* g, const monte_phantom* ph,
   if (rc) return rc;
   if ((rc = st.back(primary, dp, np * sizeof(double)))) return rc;
   return st.back(scatter, ds, np * sizeof(double));
+}
+
+int monte_mc_project_noalloc(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const monte_mc_cfg* cfg,
+                             double* primary, double* scatter, monte_mc_stats* stats) {
+  if (!c) return MONTE_ERR_INVALID;
+  if (!g || !ph || !cfg || !primary || !scatter) return fail(c, MONTE_ERR_INVALID, "null argument");
+  if (c->host_ptrs) return fail(c, MONTE_ERR_UNSUPPORTED, "monte_mc_project_noalloc takes device pointers only");
+  MONTE_HIP(c, hipSetDevice(c->device));
+  char* scratch = (char*)noalloc_scratch(c->device);
+  if (!scratch) return fail(c, MONTE_ERR_STATE, "no scratch on this device (context not created on it)");
+  const int batch = 64 * c->tuning.mc_batch;
+  std::vector<HostJob> hj(1);
+  int rc = build_job(c, g, ph, cfg, hj[0], batch);
+  if (rc) return rc;
+  const size_t need = mc_layout(hj).total;
+  if (need > kNoAllocScratchBytes)
+    return fail(c, MONTE_ERR_INVALID, "the job needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                                          std::to_string(kNoAllocScratchBytes) + " available (fewer views)");
+  hj[0].d.primary = primary;
+  hj[0].d.scatter = scatter;
+  float ms = 0.f;
+  std::vector<McCounters> cnt;
+  if ((rc = run_mc(c, hj, batch, c->tuning.mc_block, c->tuning.mc_blocks_per_cu, &ms, &cnt, scratch, -1))) return rc;
+  if (stats) fill_stats(stats, cnt[0], ms);
+  return MONTE_OK;
 }
 
 int monte_mc_first_flight_sets(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, int brick,

@@ -227,6 +227,12 @@ int fail(monte_ctx* c, int code, const std::string& msg);
 // workspace slot i grown to at least bytes; returns nullptr on failure
 void* workspace(monte_ctx* c, int slot, size_t bytes);
 
+// Device scratch of the allocation-free entry points (a static device array,
+// one per GPU; noalloc.hip): its address on `device` and its size, or nullptr
+// before noalloc_setup has run on that device.
+constexpr size_t kNoAllocScratchBytes = 128 * 1024;
+int noalloc_setup(int device);  // with `device` current; 0 or a hipError_t
+void* noalloc_scratch(int device);
 
 // ---------------------------------------------------------- kernel launches
 // FDK (fdk.hip)
