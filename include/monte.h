@@ -265,8 +265,10 @@ int monte_phantom_paint(uint8_t *labels, int nx, int ny, int nz, double vx, doub
 
 /* ------------------------------------------------------- tuning / graphs */
 /* Time candidate launch shapes on this device for the given workload and keep
- * the fastest in the context (no reference counterpart).  what: bit 1 = MC,
- * bit 2 = backprojection.  Writes the chosen values to *out if non-NULL. */
+ * the fastest in the context (no reference counterpart).  what: bit 1 = MC
+ * (block, batch, blocks per CU), bit 2 = backprojection (z chunk, tile), bit 4 =
+ * matched projector (voxels per lane).  Writes the chosen values to *out if
+ * non-NULL. */
 int monte_autotune(monte_ctx *ctx, int what, const monte_geom *g, const monte_phantom *ph,
                    const monte_mc_cfg *cfg, monte_tuning *out);
 /* Capture one full FDK (filter + backprojection over all view chunks) into a

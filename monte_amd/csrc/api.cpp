@@ -1264,6 +1264,44 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
+  if (what & 4) {
+    // the matched projector: voxels per lane along z (the automatic choice, 0,
+    // and each of 2, 4, 8) on a scratch volume, every view, after a warm-up
+    int rc = check_geom(c, g, true);
+    if (rc) return rc;
+    const size_t np = padded_elems(*g);
+    float* vol = (float*)workspace(c, kSlotTuneB, vol_elems(*g) * sizeof(float));
+    float* pad = (float*)workspace(c, kSlotAdj, np * sizeof(float));
+    if (!vol || !pad) return fail(c, MONTE_ERR_NOMEM, "tuning buffers");
+    MONTE_HIP(c, launch_fill(vol, 1.f, vol_elems(*g), c->stream));
+    hipEvent_t e0, e1;
+    MONTE_HIP(c, hipEventCreate(&e0));
+    MONTE_HIP(c, hipEventCreate(&e1));
+    const monte_tuning keep = c->tuning;
+    float best_ms = 1e30f;
+    for (int zc : {0, 2, 4, 8}) {
+      c->tuning = keep;
+      c->tuning.pj_zchunk = zc;
+      rc = run_project_padded(c, *g, vol, pad);  // warm (builds the column lists once)
+      if (!rc) {
+        (void)hipEventRecord(e0, c->stream);
+        rc = run_project_padded(c, *g, vol, pad);
+        (void)hipEventRecord(e1, c->stream);
+      }
+      if (rc) break;
+      MONTE_HIP(c, hipEventSynchronize(e1));
+      float ms = 0.f;
+      MONTE_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.pj_zchunk = zc;
+      }
+    }
+    c->tuning = keep;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+  }
   c->tuning = best;
   if (out) *out = best;
   return MONTE_OK;

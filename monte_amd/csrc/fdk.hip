@@ -395,7 +395,7 @@ __device__ inline ColumnCore column_core(const BpView& c, double X, double Y, fl
   r.kv = c.svk * inv;
   r.dsv = (float)(r.kv * g.gsz);
   // sign test in unfused FP64 (as recon/bp3d20_325.cpp:145,151 evaluates it)
-  const double gp = __dadd_rn(__dmul_rn(X, c.gx), __dmul_rn(Y, c.gy));
+  const double gp = xadd(xmul(X, c.gx), xmul(Y, c.gy));
   const float hp = fabsf(fmaf(c.hx, Xf, c.hy * Yf));
   const float dd = fmaf(c.dx, Xf, fmaf(c.dy, Yf, c.d0)) - (gp < 0.0 ? -hp : hp);
   const float q = c.wn / dd;
@@ -418,7 +418,7 @@ constexpr int kZBase = 16;
 
 __device__ inline double base_z(const BpGrid& g, int zb) {
   // recon/bp3d20_325.cpp:109 forms the coordinate as start - index * step, unfused
-  return __dadd_rn(g.gz0, __dmul_rn((double)zb, g.gsz));
+  return xadd(g.gz0, xmul((double)zb, g.gsz));
 }
 
 __device__ inline ChunkBase chunk_base(const ColumnCore& cc, const BpView& c, double Zb, const BpGrid& g) {
@@ -458,8 +458,8 @@ __device__ inline bool in_mask(const BpGrid& g, int x, int y, int z) {
 }
 
 // recon/bp3d20_325.cpp:107-108 form X and Y as start + index * step, unfused
-__device__ inline double voxel_x(const BpGrid& g, int x) { return __dadd_rn(g.gx0, __dmul_rn((double)x, g.gsx)); }
-__device__ inline double voxel_y(const BpGrid& g, int y) { return __dadd_rn(g.gy0, __dmul_rn((double)y, g.gsy)); }
+__device__ inline double voxel_x(const BpGrid& g, int x) { return xadd(g.gx0, xmul((double)x, g.gsx)); }
+__device__ inline double voxel_y(const BpGrid& g, int y) { return xadd(g.gy0, xmul((double)y, g.gsy)); }
 
 template <int ZC>
 __global__ __launch_bounds__(256) void backproject_kernel(const float4* __restrict__ quads,

@@ -12,6 +12,23 @@
 
 namespace monte {
 
+// IEEE FP64 add, subtract and multiply that the compiler never fuses into an
+// FMA, for expressions that must round exactly as a CPU evaluates them (the
+// pragma marks the operations themselves, so it holds after inlining; the
+// __dadd_rn family alone does not stop that fusion).
+__host__ __device__ inline double xadd(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__host__ __device__ inline double xsub(double a, double b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+__host__ __device__ inline double xmul(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 constexpr int kRows = MONTE_TABLE_ROWS;
 constexpr int kMaxLabels = MONTE_MAX_LABELS;
 constexpr int kPadLo = 1;  // zero guard cells before u / v in padded filtered projections

@@ -566,20 +566,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
 }
 
 // --------------------------------------------- first-flight label sets
-// IEEE FP64 add, subtract and multiply that are never fused into FMAs, so the
-// label sets follow the oracle's operations bit for bit.
-__device__ inline double xadd(double a, double b) {
-#pragma clang fp contract(off)
-  return a + b;
-}
-__device__ inline double xsub(double a, double b) {
-#pragma clang fp contract(off)
-  return a - b;
-}
-__device__ inline double xmul(double a, double b) {
-#pragma clang fp contract(off)
-  return a * b;
-}
+// (xadd, xsub and xmul are never fused into FMAs, so the label sets follow
+// the oracle's operations bit for bit.)
 
 __device__ inline uint32_t label_bit(int lab) { return lab < 4 ? 1u << lab : kAnyLabel; }
 

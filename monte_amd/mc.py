@@ -220,7 +220,8 @@ def counts_to_lineint_flat(counts: torch.Tensor, geom: Geometry, flat: torch.Ten
 
 def autotune(ph: LabelPhantom | None, geom: Geometry, settings: McSettings | None, what: int = 3,
              labels_dev: torch.Tensor | None = None, ctx: Context | None = None, device: int = 0) -> dict:
-    """Times candidate launch shapes and keeps the fastest in the context."""
+    """Times candidate launch shapes and keeps the fastest in the context.
+    what: bit 1 MC, bit 2 backprojection, bit 4 matched projector."""
     ctx = ctx or default_context(device)
     dev = torch.device("cuda", ctx.device)
     g = geom.c()

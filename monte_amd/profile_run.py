@@ -8,9 +8,10 @@
 `copy` copies M MiB device-to-device once (a known byte count, used to scale
 the FETCH_SIZE / WRITE_SIZE counters).  The others run one pass of a hot path
 (`adjoint`: matched projector + transposed filter) on the full CBCT workload (325^3 phantom at 0.5 mm, 512^2 detector at 1.2 mm,
-SOD 600 / SDD 1200 mm, 120 kVp spectrum) after one untimed warm-up pass, and
-print what ran, so the kernel of interest appears in the trace twice with the
-same arguments.
+SOD 600 / SDD 1200 mm, 120 kVp spectrum), or with `--geom e2e` on the
+end-to-end geometry (512^3 at 0.4 mm, 720 views of 1024 x 768), after one
+untimed warm-up pass, and print what ran, so the kernel of interest appears in
+the trace twice with the same arguments.
 """
 from __future__ import annotations
 
@@ -25,6 +26,8 @@ def main(argv=None):
     ap.add_argument("--views", type=int, default=360)
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--geom", choices=("full", "e2e"), default="full",
+                    help="fdk / adjoint geometry: full (325^3, 512^2) or e2e (512^3 at 0.4 mm, 1024 x 768, 720 views)")
     a = ap.parse_args(argv)
 
     import torch
@@ -43,7 +46,7 @@ def main(argv=None):
         return 0
 
     ctx = Context(0)
-    geom = geometry.full(n_views=a.views)
+    geom = geometry.full(n_views=a.views) if a.geom == "full" else geometry.e2e()
     if a.what == "mc":
         from .mc import McSettings, device_labels, mc_project
 

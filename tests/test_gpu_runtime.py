@@ -89,10 +89,10 @@ def test_autotune_records_choice(ctx):
 
     saved = ctx.tuning
     try:
-        t = autotune(phantoms.small_phantom(), G.small(n_views=4), McSettings(photons_per_view=100_000), what=3,
+        t = autotune(phantoms.small_phantom(), G.small(n_views=4), McSettings(photons_per_view=100_000), what=7,
                      ctx=ctx)
         assert t == ctx.tuning
-        assert t["bp_zchunk"] in (4, 8, 16) and t["mc_block"] in (64, 128, 256)
+        assert t["bp_zchunk"] in (4, 8, 16) and t["mc_block"] in (64, 128, 256) and t["pj_zchunk"] in (0, 2, 4, 8)
     finally:
         ctx.set_tuning(**saved)
 
