@@ -144,7 +144,7 @@ typedef struct monte_tuning {
   int32_t bp_zchunk;         /* voxels along z per backprojection thread */
   int32_t bp_tile_x, bp_tile_y; /* backprojection block tile (threads) */
   int32_t bp_view_chunk;     /* views per backprojection launch */
-  int32_t pj_zchunk;         /* projector voxels per lane along z: 2, 4, 8, or 0 = chosen from nz */
+  int32_t pj_zchunk;         /* projector voxels per lane along z: 2, 4, 8, or 0 = the default (2) */
   int32_t mc_ray_brick;      /* > 0: a photon's first flight takes as Woodcock majorant the largest mu
                                 among the labels its source -> pixel line crosses (found per view and
                                 pixel on bricks of this many voxels a side); 0: the global majorant.

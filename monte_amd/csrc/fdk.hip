@@ -692,7 +692,7 @@ __device__ inline void lds_order() { asm volatile("" ::: "memory"); }
 
 // Passes (64 chunks of C voxels) whose volume values are fetched one column
 // ahead; columns longer than kPrefetchZ / C / 64 passes load the rest in place.
-constexpr int kPrefetchZ = 384;
+constexpr int kPrefetchZ = 512;
 constexpr int kAhead = 2;
 // LDS floats after the strip that absorb the updates of lanes with nothing to
 // add (outside the v window or past the column end), so the updates need no
@@ -840,6 +840,52 @@ __device__ inline void pass_updates_pair2(float* __restrict__ strip, const float
     const float2 xa = lds_get2(strip, ia), xb = lds_get2(strip, ib);
     lds_put2(strip, ia, make_float2(xa.x + (qa - ta), xa.y + ta));
     lds_put2(strip, ib, make_float2(xb.x + (qb - tb), xb.y + tb));
+    lds_order();
+  }
+}
+
+// Every pass of an unfolded column at once (npass <= NP, one lane phase),
+// round by round: per voxel j of a lane's chunk, every pass updates the lower
+// then the upper row of its voxel.  The rows of all lanes of all passes in one
+// round are distinct (consecutive lanes, and the last lane of a pass and the
+// first of the next, are C voxels apart, so their rows differ by at least
+// C |dsv| > 1.01), so a round's reads of all passes are issued before its
+// writes and the passes' LDS latencies overlap.  Rounds follow each other in
+// program order.
+template <int C, int NP>
+__device__ inline void column_interleaved(float* __restrict__ strip, const float (&v)[NP][C], const ColumnCore& cc,
+                                          const ColumnFx& fx, int cell0, int trash, int npass, const BpGrid& g,
+                                          int lane) {
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    int i0[NP];
+    float aa[NP], a1[NP], bb[NP], b1[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int z = (p * 64 + lane) * C + j;
+      const long long sj = fx.a + (long long)z * fx.d;
+      const bool live = p < npass && sj >= fx.lo && sj <= fx.hi && z < g.nz;
+      const float sw = cc.w * v[p][j];
+      bb[p] = sw * fx_frac(sj);
+      aa[p] = sw - bb[p];
+      a1[p] = aa[p] * cc.fu;
+      b1[p] = bb[p] * cc.fu;
+      i0[p] = live ? cell0 + fx_row(sj) * kStripP : trash;
+    }
+    float2 x[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (p < npass) x[p] = lds_get2(strip, i0[p]);
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (p < npass) lds_put2(strip, i0[p], make_float2(x[p].x + (aa[p] - a1[p]), x[p].y + a1[p]));
+    lds_order();
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (p < npass) x[p] = lds_get2(strip, i0[p] + kStripP);
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (p < npass) lds_put2(strip, i0[p] + kStripP, make_float2(x[p].x + (bb[p] - b1[p]), x[p].y + b1[p]));
     lds_order();
   }
 }
@@ -1052,6 +1098,10 @@ __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ v
         return;
       }
     }
+    if (npass <= NP && phases == 1) {  // every pass of an unfolded column at once
+      column_interleaved<C, NP>(strip, vals.v, cc, fx, cell0, trash + 2 * lane, npass, g, lane);
+      return;
+    }
 #pragma unroll
     for (int p = 0; p < NP; ++p)
       if (p < npass) project_pass<C>(strip, vals.v[p], cc, fx, cell0, trash + 2 * lane, p * 64, phases, g, lane);
@@ -1215,15 +1265,11 @@ hipError_t launch_project(const float* volT, const BpView* views, int view0, int
   if (lds_r0 < 0 || lds_rows < 1 || lds_r0 + lds_rows > grid.rows) return hipErrorInvalidValue;
   const size_t lds = sizeof(float) * ((size_t)lds_rows * kStripP + kTrash);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
-  // voxels per lane: fill the wave's 64 lanes best over nz, favouring short chunks
-  int best = t.pj_zchunk;
-  long best_cost = -1;
-  for (int cz : {2, 4, 8}) {
-    if (t.pj_zchunk) break;
-    const long passes = ((grid.nz + cz - 1) / cz + 63) / 64;
-    const long cost = passes * (cz * 30 + 20);
-    if (best_cost < 0 || cost < best_cost) { best = cz; best_cost = cost; }
-  }
+  // voxels per lane: 2 unless tuned otherwise.  Two voxels per lane need the
+  // fewest dependent LDS read-add-writes per pass: 2 and 4 voxels per lane
+  // took 10.6 and 13.8 ms at 325^3 / 512^2 / 360 views, 86 and 96 ms at
+  // 512^3 / 1024 x 768 / 720 views (and 8 voxels per lane 163 ms).
+  const int best = t.pj_zchunk ? t.pj_zchunk : 2;
   const dim3 gd((unsigned)nblocks), bd(64);
 #define MONTE_PJ(CZ, MK)                                                                                      \
   hipLaunchKernelGGL((project_kernel<CZ, MK>), gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad, \

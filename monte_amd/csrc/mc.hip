@@ -41,8 +41,10 @@ __device__ inline void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32
     const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
     const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
     const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-    const uint32_t n0 = hi1 ^ c1 ^ k0;
-    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    // hi ^ c ^ k as one three-input XOR (truth table 0x96)
+    uint32_t n0, n2;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(n0) : "v"(hi1), "v"(c1), "s"(k0));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(n2) : "v"(hi0), "v"(c3), "s"(k1));
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
     k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
   }
@@ -73,50 +75,83 @@ __device__ inline float next_u01(Stream& s, uint32_t job, uint32_t k0, uint32_t 
   return u01(x);
 }
 
+// The stream's next three uniforms, in order, with at most one new block: the
+// same numbers as three next_u01 calls, but with one Philox call site, so a
+// wave whose lanes sit at different offsets in their blocks runs the rounds
+// once per trial of a rejection loop rather than once per draw.
+__device__ inline void next3_u01(Stream& s, uint32_t job, uint32_t k0, uint32_t k1, float& r1, float& r2, float& r3) {
+  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  const int left = s.left;
+  if (left < 3) {
+    n0 = s.ctr; n1 = s.photon; n2 = s.view; n3 = job;
+    philox10(n0, n1, n2, n3, k0, k1);
+    s.ctr += 1;
+  }
+  // the unconsumed words in order are b0 .. b(left-1), then n0 .. n3
+  const uint32_t x0 = left == 0 ? n0 : s.b0;
+  const uint32_t x1 = left == 0 ? n1 : (left == 1 ? n0 : s.b1);
+  const uint32_t x2 = left == 0 ? n2 : (left == 1 ? n1 : (left == 2 ? n0 : s.b2));
+  // and what stays: left + 1 words after a new block, left - 3 otherwise
+  const uint32_t w0 = left == 0 ? n3 : (left == 1 ? n2 : (left == 2 ? n1 : s.b3));
+  const uint32_t w1 = left == 1 ? n3 : n2;
+  s.b0 = w0; s.b1 = w1; s.b2 = n3;
+  s.left = left < 3 ? left + 1 : left - 3;
+  r1 = u01(x0); r2 = u01(x1); r3 = u01(x2);
+}
+
+// Reciprocal to 1 ulp (v_rcp_f32).  The physics below draws angles and
+// energies whose parity with the FP64 oracle is statistical, so it takes the
+// hardware reciprocal, square root and sine instead of correctly rounded
+// sequences.
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // --------------------------------------------------------------- physics
 // Kahn's rejection sampling of the Klein-Nishina distribution as written in
-// CBCT_real325.cu:687-722; returns rho = lambda'/lambda.
-__device__ inline float kahn(float e_kev, Stream& s, uint32_t job, uint32_t k0, uint32_t k1) {
-  const float lambda = 511.0f / e_kev;
-  const float p1 = (lambda + 2.0f) / (9.0f * lambda + 2.0f);
+// CBCT_real325.cu:687-722 (lambda = 511/E); returns rho = lambda'/lambda.
+// Each trial takes the stream's next three uniforms (r1, r2, r3) whichever
+// branch r1 picks, as the reference's trials do.
+__device__ inline float kahn(float lambda, Stream& s, uint32_t job, uint32_t k0, uint32_t k1) {
+  const float il2 = 2.0f * rcp(lambda);
+  const float p1 = (lambda + 2.0f) * rcp(9.0f * lambda + 2.0f);
   for (int guard = 0; guard < 1000; ++guard) {
-    const float r1 = next_u01(s, job, k0, k1);
+    float r1, r2, r3;
+    next3_u01(s, job, k0, k1, r1, r2, r3);
     if (r1 < p1) {
-      const float r2 = next_u01(s, job, k0, k1);
-      const float ro = 1.0f + (2.0f / lambda) * r2;
-      const float r3 = next_u01(s, job, k0, k1);
-      if (r3 <= 4.0f * ((1.0f / ro) - (1.0f / (ro * ro)))) return ro;
+      const float ro = 1.0f + il2 * r2;
+      const float iro = rcp(ro);
+      if (r3 <= 4.0f * (iro - iro * iro)) return ro;
     } else {
-      const float r2 = next_u01(s, job, k0, k1);
-      const float ro = (lambda + 2.0f) / (lambda + 2.0f * (1.0f - r2));
-      const float r3 = next_u01(s, job, k0, k1);
+      const float ro = (lambda + 2.0f) * rcp(lambda + 2.0f * (1.0f - r2));
       const float q = lambda - ro * lambda + 1.0f;
-      if (r3 <= 0.5f * (q * q + (1.0f / ro))) return ro;
+      if (r3 <= 0.5f * (q * q + rcp(ro))) return ro;
     }
   }
   return 1.0f;  // unreachable in practice (acceptance > 1/2 per trial)
 }
 
-// Thomson angular distribution (1 + mu^2): closed-form inverse CDF.
+// Thomson angular distribution (1 + mu^2): closed-form inverse CDF
+// mu = cbrt(A) - cbrt(1/A), A = |a| + sqrt(a^2 + 1) >= 1, a = 4u - 2, with
+// cbrt(A) = 2^(log2(A) / 3) on the hardware log and exp.
 __device__ inline float thomson_mu(float u) {
   const float a = 4.0f * u - 2.0f;
-  const float A = fabsf(a) + sqrtf(a * a + 1.0f);
-  const float m = cbrtf(A) - cbrtf(1.0f / A);
+  const float A = fabsf(a) + __builtin_amdgcn_sqrtf(a * a + 1.0f);
+  const float c = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(A) * 0.333333343f);
+  const float m = c - rcp(c);
   return a < 0.f ? -m : m;
 }
 
-// Rotate unit direction d by polar cosine c and azimuth phi: c d + s (cos(phi)
-// e1 + sin(phi) e2) with e1 = (dx dz, dy dz, -r^2)/r, e2 = (-dy, dx, 0)/r and
+// Rotate unit direction d by polar cosine c and azimuth phi = 2 pi w (w in
+// revolutions, as the hardware sine takes it): c d + s (cos(phi) e1 +
+// sin(phi) e2) with e1 = (dx dz, dy dz, -r^2)/r, e2 = (-dy, dx, 0)/r and
 // r = sqrt(dx^2 + dy^2), which stays accurate near the poles (1 - dz^2 would
 // cancel); only for d on the z axis is the pair the x and y axes.
-__device__ inline void rotate_dir(float& dx, float& dy, float& dz, float c, float phi) {
-  const float s = sqrtf(fmaxf(0.f, 1.f - c * c));
-  float sp, cp;
-  sincosf(phi, &sp, &cp);
-  const float r = sqrtf(dx * dx + dy * dy);
+__device__ inline void rotate_dir(float& dx, float& dy, float& dz, float c, float w) {
+  const float s = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - c * c));
+  const float sp = __builtin_amdgcn_sinf(w), cp = __builtin_amdgcn_cosf(w);
+  const float r = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
   float nx, ny, nz;
   if (r > 1e-15f) {  // r^2 stays a normal FP32 number
-    const float ir = 1.0f / r;
+    const float ir = rcp(r);
     nx = c * dx + s * (dx * dz * cp - dy * sp) * ir;
     ny = c * dy + s * (dy * dz * cp + dx * sp) * ir;
     nz = c * dz - s * r * cp;
@@ -198,11 +233,25 @@ struct Photon {
 
 // Per-wave LDS copy of the current job's per-energy tables, read at every
 // birth and flight start (the spectrum search alone is 8 dependent reads).
+constexpr int kGuide = 256;  // spectrum guide cells
 struct WaveTables {
   float cdf[kRows];      // spectrum CDF (unused for mono jobs)
   float mumax[kRows];    // Woodcock majorant
   float mu[4][kRows];    // mu of labels 0..3
+  // guide[g] = the energy row drawn for u = g / kGuide (rows are <= 200): a
+  // draw u in [g, g + 1) / kGuide lies between guide[g] and guide[g + 1]
+  uint8_t guide[kGuide + 4];
 };
+
+// Inverse CDF of the spectrum (CBCT_real325.cu:479-486): the first row e in
+// [1, kRows - 1] with cdf[e] >= u, or the last row, searched within [lo, hi].
+__device__ inline int spectrum_row(const float* cdf, float u, int lo, int hi) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] < u) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
 
 enum { kDead = 0, kFly = 1, kCollide = 2 };
 // Per-lane statistics, counted in the branches where the events happen and
@@ -281,17 +330,26 @@ __device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables&
   p.t = 0.f;
   p.t1 = t1;
   p.mumax = mm;
-  p.lscale = 0.69314718055994531f / mm;
+  p.lscale = 0.69314718055994531f * rcp(mm);
   return true;
 }
 
 // Copies a job's tables into the wave's LDS slot.
+// The host builds the CDF from non-negative weights, so it is non-decreasing
+// and the search within a guide cell returns the row a search over all rows
+// would.
 __device__ inline void load_wave_tables(WaveTables& wt, const McJob& J, int lane) {
   for (int e = lane; e < kRows; e += 64) {
     wt.cdf[e] = J.spectrum_cdf ? J.spectrum_cdf[e] : 0.f;
     wt.mumax[e] = J.tables->mumax[e];
 #pragma unroll
     for (int l = 0; l < 4; ++l) wt.mu[l][e] = J.tables->m[l][e].x;
+  }
+  if (J.spectrum_cdf) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the CDF in LDS is complete
+    __builtin_amdgcn_wave_barrier();
+    for (int g = lane; g <= kGuide; g += 64)
+      wt.guide[g] = (uint8_t)spectrum_row(wt.cdf, (float)g * (1.0f / kGuide), 1, kRows - 1);
   }
 }
 
@@ -343,6 +401,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
                                                            McCounters* __restrict__ counters, int batch) {
   int job = -1;
   uint32_t next = 0, end = 0;  // photon range of the current unit (wave-uniform)
+  uint32_t cur_iu = 0, cur_iv = 0;  // pixel photon `next` aims at (wave-uniform), so births need no division
   int unit_view = 0;
   long long pending = -1;
   bool exhausted = false;
@@ -414,6 +473,11 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           unit_view = __builtin_amdgcn_readfirstlane(cjobs[pj].view_begin + (int)vl);
           next = (uint32_t)__builtin_amdgcn_readfirstlane((int)(bi * (uint32_t)batch));
           end = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(next + (uint32_t)batch, cjobs[pj].photons_per_view));
+          {
+            const uint32_t nu_j = (uint32_t)cjobs[pj].nu, pix0 = next % (nu_j * (uint32_t)cjobs[pj].nv);
+            cur_iv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(pix0 / nu_j));
+            cur_iu = (uint32_t)__builtin_amdgcn_readfirstlane((int)(pix0 - cur_iv * nu_j));
+          }
           pending = -1;
           if (switched) {  // J is re-read for the new job at the top of the loop
             load_wave_tables(wt, load_job(cjobs, pj), lane);
@@ -434,10 +498,17 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         if (id < end) {
           ++n_hist;
           rs.ctr = 0; rs.photon = id; rs.view = (uint32_t)unit_view; rs.left = 0;
-          // aim at the centre of pixel id mod (nu*nv) (CBCT_real325.cu:464-466)
+          // aim at the centre of pixel id mod (nu*nv) (CBCT_real325.cu:464-466), in the
+          // order iv * nu + iu (u fastest, so a batch covers whole detector rows):
+          // stepped on from the wave's cursor, id - next = rank < 64 pixels on
           const uint32_t npix = (uint32_t)(Jp->nu * Jp->nv);
-          const uint32_t pix = id % npix;  // aim order: u fastest, so a batch covers whole detector rows
-          const int iv = (int)(pix / (uint32_t)Jp->nu), iu = (int)(pix - (uint32_t)iv * (uint32_t)Jp->nu);
+          uint32_t u_ = cur_iu + rank, v_ = cur_iv;
+          while (u_ >= (uint32_t)Jp->nu) {
+            u_ -= (uint32_t)Jp->nu;
+            v_ = v_ + 1 == (uint32_t)Jp->nv ? 0u : v_ + 1;
+          }
+          const int iu = (int)u_, iv = (int)v_;
+          const uint32_t pix = v_ * (uint32_t)Jp->nu + u_;
           const float yp = ((float)iu + 0.5f - Jp->un0) * Jp->u_step;
           const float zp = ((float)iv + 0.5f - Jp->vn0) * Jp->v_step;
           const float in = rsqrtf(Jp->sdd * Jp->sdd + yp * yp + zp * zp);
@@ -453,12 +524,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           float e = Jp->e_kev;
           if (Jp->spectrum_cdf) {
             const float u = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
-            int lo = 1, hi = kRows - 1;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (wt.cdf[mid] < u) lo = mid + 1; else hi = mid;
-            }
-            e = (float)lo;
+            const int g = (int)(u * (float)kGuide);  // exact: u < 1 and kGuide is a power of 2
+            e = (float)spectrum_row(wt.cdf, u, wt.guide[g], wt.guide[g + 1]);
           }
           p.e = e;
           p.eb = ebin_of(e);
@@ -474,7 +541,17 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           }
         }
       }
-      next = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(end, next + popc(dead)));
+      {
+        const uint32_t nxt = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(end, next + popc(dead)));
+        uint32_t u_ = cur_iu + (nxt - next), v_ = cur_iv;
+        while (u_ >= (uint32_t)Jp->nu) {
+          u_ -= (uint32_t)Jp->nu;
+          v_ = v_ + 1 == (uint32_t)Jp->nv ? 0u : v_ + 1;
+        }
+        cur_iu = (uint32_t)__builtin_amdgcn_readfirstlane((int)u_);
+        cur_iv = (uint32_t)__builtin_amdgcn_readfirstlane((int)v_);
+        next = nxt;
+      }
     }
 
     // 3) interactions of the parked colliding lanes
@@ -496,19 +573,20 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             ++n_coh;
             if (Jp->coherent) {
               const float cth = thomson_mu(next_u01(rs, Jp->job_id, Jp->key0, Jp->key1));
-              const float phi = 6.283185307179586f * next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
-              rotate_dir(p.dx, p.dy, p.dz, cth, phi);
+              const float w = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);  // azimuth / 2 pi
+              rotate_dir(p.dx, p.dy, p.dz, cth, w);
             }
           } else {
             ++n_comp;
-            const float ro = kahn(p.e, rs, Jp->job_id, Jp->key0, Jp->key1);
-            const float lam = 511.0f / p.e, lamd = ro * lam;
+            const float lam = 511.0f * rcp(p.e);
+            const float ro = kahn(lam, rs, Jp->job_id, Jp->key0, Jp->key1);
+            const float lamd = ro * lam;
             float cth = 1.0f - (lamd - lam);
             cth = cth < -1.f ? -1.f : cth;
-            p.e = 511.0f / lamd;
+            p.e = 511.0f * rcp(lamd);
             p.eb = ebin_of(p.e);
-            const float phi = 6.283185307179586f * next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
-            rotate_dir(p.dx, p.dy, p.dz, cth, phi);
+            const float w = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);  // azimuth / 2 pi
+            rotate_dir(p.dx, p.dy, p.dz, cth, w);
           }
           p.nint += 1;
           if (start_flight(p, *Jp, wt, kAnyLabel)) {

@@ -39,11 +39,16 @@ def _geoms():
     # a v-slope of 0.8 rows per voxel, as in the 325^3 / 512^2 workload: two
     # voxels per lane fold into three row updates
     mid = std.replace(nz=45, vz=0.04, n_views=12)
+    # tall columns: the projector walks several 64-lane passes per column and
+    # runs their LDS updates round by round together, folded (v-slope 0.8) and
+    # unfolded (1.1 rows per voxel)
+    tall = std.replace(nx=24, ny=24, nz=300, vz=0.04, nv=256, n_views=8)
+    tall1 = std.replace(nx=24, ny=24, nz=400, vz=0.055, nv=480, n_views=8)
     return {"std": std, "shepp": shepp, "bp3d20": compat, "bp3d20_mask": compat_mask, "thin": thin,
-            "thinner": thinner, "mid": mid}
+            "thinner": thinner, "mid": mid, "tall": tall, "tall1": tall1}
 
 
-@pytest.mark.parametrize("name", ["std", "shepp", "bp3d20", "bp3d20_mask", "thin", "thinner", "mid"])
+@pytest.mark.parametrize("name", ["std", "shepp", "bp3d20", "bp3d20_mask", "thin", "thinner", "mid", "tall", "tall1"])
 def test_dot_product_identity(ctx, name):
     from monte_amd.fdk import (fdk, fdk_adjoint, fdk_backproject, fdk_filter, fdk_filter_adjoint,
                                fdk_project)
@@ -68,7 +73,7 @@ def test_dot_product_identity(ctx, name):
     assert abs(f1 - f2) <= 1e-5 * abs(f1)
 
 
-@pytest.mark.parametrize("name", ["std", "thinner", "mid"])
+@pytest.mark.parametrize("name", ["std", "thinner", "mid", "tall", "tall1"])
 def test_adjoint_matches_oracle(ctx, oracle, name):
     from monte_amd.fdk import fdk_adjoint
 
@@ -138,7 +143,7 @@ def test_autograd_batch_and_graph_of_ops(ctx):
     assert torch.allclose(x.grad, expect, rtol=1e-6, atol=1e-6 * expect.abs().max().item())
 
 
-@pytest.mark.parametrize("name", ["std", "thin", "mid", "bp3d20_mask"])
+@pytest.mark.parametrize("name", ["std", "thin", "mid", "bp3d20_mask", "tall", "tall1"])
 def test_projector_agrees_across_lane_chunks(ctx, name):
     """Every voxels-per-lane variant of the projector is the same transpose:
     each passes the dot-product identity, and they agree with each other up
@@ -164,7 +169,7 @@ def test_projector_agrees_across_lane_chunks(ctx, name):
         assert rel_max(outs[cz], outs[2]) < 1e-6, cz
 
 
-@pytest.mark.parametrize("name", ["std", "bp3d20"])
+@pytest.mark.parametrize("name", ["std", "bp3d20", "tall1"])
 def test_projector_is_deterministic(ctx, name):
     """Repeated projections are bitwise equal: the strip edge columns take two
     addends onto zero, and the bp3d20 guard cells are folded back by a gather
