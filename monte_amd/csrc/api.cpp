@@ -403,7 +403,7 @@ McLayout mc_layout(const std::vector<HostJob>& hj) {
     L.cdf_off[i] = off; off = align_up(off + sizeof(float) * hj[i].cdf.size(), 256);
     L.cs_off[i] = off; off = align_up(off + sizeof(float2) * hj[i].view_cs.size(), 256);
   }
-  L.q_off = off; off = align_up(off + 256, 256);
+  L.q_off = off; off = align_up(off + kMcQueueBytes, 256);
   L.cnt_off = off; off = align_up(off + sizeof(McCounters) * n, 256);
   L.box_off = off; off = align_up(off + sizeof(int) * 8 * n, 256);
   L.total = off;
@@ -543,6 +543,7 @@ int prepare_mc(monte_ctx* c, std::vector<HostJob>& hj, int block, int blocks_per
   P.L = Lo;
   P.n = n;
   P.units = units;
+  P.chunk = (uint32_t)std::max(1, hj[0].d.n_views_sim);
   return MONTE_OK;
 }
 
@@ -554,7 +555,8 @@ hipError_t launch_prepared(monte_ctx* c, const McPrepared& P, int batch, int blo
   if (e != hipSuccess) return e;
   if (start && (e = hipEventRecord(start, c->stream)) != hipSuccess) return e;
   const int grid = std::max(1, c->n_cu * blocks_per_cu);
-  return launch_mc_transport((const McJob*)(P.base + P.L.jobs_off), P.n, P.units, P.queue(), P.counters(), batch,
+  return launch_mc_transport((const McJob*)(P.base + P.L.jobs_off), P.n, P.units, P.chunk, P.queue(), P.counters(),
+                             batch,
                              block, grid, c->stream);
 }
 

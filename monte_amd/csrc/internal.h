@@ -171,6 +171,7 @@ struct McPrepared {
   McLayout L;
   int n = 0;
   uint64_t units = 0;
+  uint32_t chunk = 1;  // units per queue chunk (the first job's simulated views)
   unsigned int* queue() const { return (unsigned int*)(base + L.q_off); }
   McCounters* counters() const { return (McCounters*)(base + L.cnt_off); }
 };
@@ -291,8 +292,16 @@ hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* 
                                hipStream_t s);
 
 // MC (mc.hip)
-hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, unsigned int* queue,
-                               McCounters* counters, int batch, int block, int grid, hipStream_t s);
+// The work queue is kMcQueues counters kMcQueueStride words apart: queue q
+// hands out the chunks q, q + kMcQueues, ... of `chunk` consecutive work
+// units, and block b draws from queue b mod kMcQueues first (blocks are
+// dealt to the XCDs round robin, so each XCD's L2 serves its own chunks).
+constexpr int kMcQueues = 8;
+constexpr int kMcQueueStride = 16;
+constexpr size_t kMcQueueBytes = sizeof(unsigned int) * kMcQueues * kMcQueueStride;
+hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, uint32_t chunk,
+                               unsigned int* queue, McCounters* counters, int batch, int block, int grid,
+                               hipStream_t s);
 hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs, hipStream_t s);
 // First-flight label sets: per brick of b^3 voxels widened by one voxel
 // (bricks [nb2][nb1][nb0]), then per view of [view_begin, view_begin +

@@ -397,9 +397,12 @@ constexpr unsigned kInteractBatch = 24;
 constexpr unsigned kRefillBatch = 16;
 
 __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restrict__ jobs, int n_jobs,
-                                                           uint64_t total_units, unsigned int* __restrict__ queue,
+                                                           uint64_t total_units, uint32_t chunk,
+                                                           unsigned int* __restrict__ queue,
                                                            McCounters* __restrict__ counters, int batch) {
   int job = -1;
+  const int q_first = (int)(blockIdx.x % kMcQueues);
+  int q_tried = 0;  // queues found spent (the wave then draws from the next)
   uint32_t next = 0, end = 0;  // photon range of the current unit (wave-uniform)
   uint32_t cur_iu = 0, cur_iv = 0;  // pixel photon `next` aims at (wave-uniform), so births need no division
   int unit_view = 0;
@@ -451,10 +454,21 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
     // 1) secure a work unit when lanes are idle and the current one is spent
     if (dead != 0 && next >= end) {
       if (pending < 0 && !exhausted) {
-        unsigned int u = 0;
-        if (lane == 0) u = atomicAdd(queue, 1u);
-        u = (unsigned int)__builtin_amdgcn_readfirstlane((int)u);
-        if ((uint64_t)u >= total_units) exhausted = true; else pending = (long long)u;
+        // the k-th unit of queue q is unit (k / chunk kMcQueues + q) chunk + k mod chunk;
+        // that increases with k, so a queue is spent once it hands out a unit past the end
+        while (q_tried < kMcQueues) {
+          const int q = (q_first + q_tried) % kMcQueues;
+          unsigned int k = 0;
+          if (lane == 0) k = atomicAdd(queue + q * kMcQueueStride, 1u);
+          k = (unsigned int)__builtin_amdgcn_readfirstlane((int)k);
+          const uint64_t u = ((uint64_t)(k / chunk) * kMcQueues + (uint64_t)q) * chunk + (k % chunk);
+          if (u < total_units) {
+            pending = (long long)u;
+            break;
+          }
+          ++q_tried;
+        }
+        if (pending < 0) exhausted = true;
       }
       if (pending >= 0) {
         const int pj = __builtin_amdgcn_readfirstlane(job_of_unit(cjobs, n_jobs, (uint64_t)pending));
@@ -760,7 +774,7 @@ hipError_t launch_ray_sets(const RayGrid& R, const double* cs64, int view_begin,
 
 __global__ void mc_reset_kernel(unsigned int* queue, McCounters* counters, int n_jobs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *queue = 0u;
+  if (i < kMcQueues) queue[i * kMcQueueStride] = 0u;
   if (i < n_jobs) counters[i] = McCounters{};
 }
 
@@ -822,11 +836,12 @@ hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs
   return hipGetLastError();
 }
 
-hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, unsigned int* queue,
-                               McCounters* counters, int batch, int block, int grid, hipStream_t s) {
-  if (block % 64 || block > 256 || block <= 0 || grid <= 0) return hipErrorInvalidValue;
+hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, uint32_t chunk,
+                               unsigned int* queue, McCounters* counters, int batch, int block, int grid,
+                               hipStream_t s) {
+  if (block % 64 || block > 256 || block <= 0 || grid <= 0 || chunk == 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(mc_transport_kernel, dim3(grid), dim3(block), sizeof(WaveTables) * (block / 64), s, jobs_dev,
-                     n_jobs, total_units, queue, counters, batch);
+                     n_jobs, total_units, chunk, queue, counters, batch);
   return hipGetLastError();
 }
 
