@@ -639,8 +639,13 @@ int e2e_launches(monte_ctx* c, const monte_geom& g, const monte_e2e_buffers& b, 
 monte_tuning default_tuning() {
   monte_tuning t{};
   t.mc_block = 256;
-  t.mc_batch = 32;          // with 5 blocks per CU, fastest at 325^3 / 512^2 / 360 views x 1e7 photons
-  t.mc_blocks_per_cu = 5;
+  // 4 blocks of 256 per CU is the occupancy of the transport kernel (4 waves per
+  // SIMD).  With 32 work queues, 64 histories per lane per unit was fastest at
+  // 325^3 / 512^2 / 360 views x 1e7 photons (158 ms; 164 and 166 ms for 32 and
+  // 128): fewer, larger units contend less on the queue counters, more keep
+  // each XCD's units on fewer detector rows.
+  t.mc_batch = 64;
+  t.mc_blocks_per_cu = 4;
   t.bp_zchunk = 16;
   t.bp_tile_x = 8;   // fastest at 325^3 / 512^2 / 360 views
   t.bp_tile_y = 32;

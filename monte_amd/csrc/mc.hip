@@ -454,14 +454,18 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
     // 1) secure a work unit when lanes are idle and the current one is spent
     if (dead != 0 && next >= end) {
       if (pending < 0 && !exhausted) {
-        // the k-th unit of queue q is unit (k / chunk kMcQueues + q) chunk + k mod chunk;
-        // that increases with k, so a queue is spent once it hands out a unit past the end
+        // in round r = k / chunk, queue q hands out chunk r kMcQueues + (q + r) mod kMcQueues,
+        // unit k mod chunk of it: the chunks of a round are dealt to the queues rotated by
+        // one per round, so no queue keeps the same detector rows when kMcQueues chunks
+        // span a whole number of detector sweeps.  Units increase with k, so a queue is
+        // spent once it hands out a unit past the end.
         while (q_tried < kMcQueues) {
           const int q = (q_first + q_tried) % kMcQueues;
           unsigned int k = 0;
           if (lane == 0) k = atomicAdd(queue + q * kMcQueueStride, 1u);
           k = (unsigned int)__builtin_amdgcn_readfirstlane((int)k);
-          const uint64_t u = ((uint64_t)(k / chunk) * kMcQueues + (uint64_t)q) * chunk + (k % chunk);
+          const uint32_t r = k / chunk;
+          const uint64_t u = ((uint64_t)r * kMcQueues + (uint64_t)((q + r) % kMcQueues)) * chunk + (k - r * chunk);
           if (u < total_units) {
             pending = (long long)u;
             break;
