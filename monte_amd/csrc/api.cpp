@@ -1191,13 +1191,16 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
   monte_tuning best = c->tuning;
   if (what & 1) {
     if (!g || !ph || !cfg) return fail(c, MONTE_ERR_INVALID, "MC tuning needs geometry, phantom and config");
-    // a short run over all the job's views (units run views fastest, so the
-    // view count shapes the memory access pattern), about 2^23 photons in all
-    // and at least 2^16 per view, into scratch tallies
+    // a run over all the job's views (units run views fastest, so the view
+    // count shapes the memory access pattern), about 2^28 photons in all and at
+    // least 2^16 per view, into scratch tallies: a launch of the full workload
+    // runs long enough that its last units' tail hardly counts, and the runs
+    // timed here must too (at 2^23 photons, launches of one wave per SIMD won
+    // and were 2.5x slower on the full workload)
     monte_mc_cfg tc = *cfg;
     int vb = cfg->view_begin, ve = cfg->view_end;
     if (vb == 0 && ve == 0) ve = g->n_views;
-    const uint64_t per_view = std::max<uint64_t>(1u << 16, (1u << 23) / (uint64_t)std::max(ve - vb, 1));
+    const uint64_t per_view = std::max<uint64_t>(1u << 16, (1ull << 28) / (uint64_t)std::max(ve - vb, 1));
     tc.photons_per_view = std::min<uint64_t>(cfg->photons_per_view, per_view);
     tc.view_begin = vb;
     tc.view_end = ve;
@@ -1206,12 +1209,13 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
     if (!tally) return fail(c, MONTE_ERR_NOMEM, "tuning tallies");
     float best_ms = 1e30f;
     const int blocks[] = {64, 128, 256};
-    const int batches[] = {16, 32, 48, 64};
-    const int bpcs[] = {2, 4, 5, 8};
+    const int batches[] = {32, 64, 128};
+    const int waves_per_cu[] = {8, 16, 20};  // 16 is the kernel's occupancy (4 waves per SIMD)
     for (int blk : blocks)
       for (int bt : batches)
-        for (int bpc : bpcs) {
-          if (blk * bpc > 2048) continue;
+        for (int wpc : waves_per_cu) {
+          const int bpc = wpc / (blk / 64);
+          if (bpc < 1 || blk * bpc > 2048) continue;
           HostJob hj;
           int rc = build_job(c, g, ph, &tc, hj, 64 * bt);
           if (rc) return rc;

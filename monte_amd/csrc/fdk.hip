@@ -694,6 +694,10 @@ __device__ inline void lds_order() { asm volatile("" ::: "memory"); }
 // ahead; columns longer than kPrefetchZ / C / 64 passes load the rest in place.
 constexpr int kPrefetchZ = 512;
 constexpr int kAhead = 2;
+// Views per group of the projector's block order (see project_kernel): 4, 8
+// and 16 took 10.4, 10.0 and 10.4 ms at 325^3 / 360 views (51 -> 35, 26 and
+// 29 GB of fabric reads), 76.5, 74.1 and 75.3 ms at 512^3 / 720 views.
+constexpr int kProjViewGroup = 8;
 // LDS floats after the strip that absorb the updates of lanes with nothing to
 // add (outside the v window or past the column end), so the updates need no
 // branches: two per lane, two row pitches and one cell of overlap.  (Branches
@@ -1049,13 +1053,24 @@ __global__ __launch_bounds__(64) void project_plan_kernel(const BpView* __restri
 template <int C, bool MASK>
 __global__ __launch_bounds__(64) void project_kernel(const float* __restrict__ volT,
                                                      const BpView* __restrict__ views, int view0, int n_strips,
-                                                     BpGrid g, float* __restrict__ filt, int lds_r0, int lds_rows,
+                                                     int nviews, BpGrid g, float* __restrict__ filt, int lds_r0,
+                                                     int lds_rows,
                                                      const uint32_t* __restrict__ offs,
                                                      const uint32_t* __restrict__ list) {
   // [lds_rows][kStripP] (padded rows lds_r0.., column q = padded column pu0 + q - kStripC0), then kTrash floats
   extern __shared__ float strip[];
-  const int k = view0 + blockIdx.x / n_strips;
-  const int st = blockIdx.x - (k - view0) * n_strips;
+  // Block b runs on XCD b mod 8 (blocks are dealt to the XCDs round robin):
+  // XCD x takes the views [x V / 8, (x + 1) V / 8) of the launch, in groups of
+  // kProjViewGroup views, strip by strip within a group.  A voxel column lands
+  // in nearby strips in nearby views, so the group's waves that read it run
+  // together on one XCD and its L2 serves the column to most of them.
+  const int xcd = (int)(blockIdx.x % 8u), jx = (int)(blockIdx.x / 8u);
+  const int va = xcd * nviews / 8, vx = (xcd + 1) * nviews / 8 - va;
+  if (jx >= vx * n_strips) return;  // (the whole one-wave block)
+  const int grp = jx / (kProjViewGroup * n_strips), rem = jx - grp * kProjViewGroup * n_strips;
+  const int gsize = min(kProjViewGroup, vx - grp * kProjViewGroup);
+  const int st = rem / gsize;
+  const int k = view0 + va + grp * kProjViewGroup + (rem - st * gsize);
   const int lane = threadIdx.x;
   const int trash = lds_rows * kStripP;
   for (int i = lane; i < trash + kTrash; i += 64) strip[i] = 0.f;
@@ -1259,7 +1274,8 @@ hipError_t launch_project(const float* volT, const BpView* views, int view0, int
                           float* filt_pad, int lds_r0, int lds_rows, const uint32_t* plan_offs,
                           const uint32_t* plan_list, const monte_tuning& t, hipStream_t s) {
   const int n_strips = project_strips(grid);
-  const long nblocks = (long)n_strips * nviews_chunk;
+  // (8 XCDs x the largest view share x strips; blocks past an XCD's share return)
+  const long nblocks = (long)n_strips * 8 * ((nviews_chunk + 7) / 8);
   if (nblocks <= 0) return hipSuccess;
   if (nblocks > 0x7fffffffL) return hipErrorInvalidValue;
   if (lds_r0 < 0 || lds_rows < 1 || lds_r0 + lds_rows > grid.rows) return hipErrorInvalidValue;
@@ -1272,7 +1288,8 @@ hipError_t launch_project(const float* volT, const BpView* views, int view0, int
   const int best = t.pj_zchunk ? t.pj_zchunk : 2;
   const dim3 gd((unsigned)nblocks), bd(64);
 #define MONTE_PJ(CZ, MK)                                                                                      \
-  hipLaunchKernelGGL((project_kernel<CZ, MK>), gd, bd, lds, s, volT, views, view0, n_strips, grid, filt_pad, \
+  hipLaunchKernelGGL((project_kernel<CZ, MK>), gd, bd, lds, s, volT, views, view0, n_strips, nviews_chunk, grid,   \
+                     filt_pad,                                                                                  \
                      lds_r0, lds_rows, plan_offs, plan_list)
   const bool mk = grid.mask != 0;
   switch (best) {
