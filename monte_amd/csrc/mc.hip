@@ -306,8 +306,10 @@ constexpr uint32_t kAnyLabel = 16u;
 // (caller tallies it).  The Woodcock majorant is the largest mu among the
 // labels of `set` (bit l: label l in 0..3; kAnyLabel: every label).
 __device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables& wt, uint32_t set) {
-  float t0, t1;
-  if (!ray_box(p.px, p.py, p.pz, p.dx, p.dy, p.dz, J, t0, t1) || !(t1 > 0.f)) return false;
+  // The majorant is formed before the box test so that `set` (a global load at a
+  // birth) is consumed on every path: a load left pending on the early-exit path
+  // made the compiler wait for all memory operations, tally atomics included,
+  // at the start of every Woodcock step.
   p.mu0 = wt.mu[0][p.eb];
   p.mu1 = wt.mu[1][p.eb];
   p.mu2 = wt.mu[2][p.eb];
@@ -323,6 +325,8 @@ __device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables&
     if (set & 8u) mm = fmaxf(mm, p.mu3);
   }
   if (!(mm > 0.f)) return false;
+  float t0, t1;
+  if (!ray_box(p.px, p.py, p.pz, p.dx, p.dy, p.dz, J, t0, t1) || !(t1 > 0.f)) return false;
   if (t0 > 0.f) {
     p.px = fmaf(t0, p.dx, p.px); p.py = fmaf(t0, p.dy, p.py); p.pz = fmaf(t0, p.dz, p.pz);
     t1 -= t0;
@@ -582,12 +586,13 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         const float4 m = Jp->tables->m[p.lab][p.eb];
         rs.left = 0;  // interactions start on a fresh block
         const float u3 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+        const bool photo = u3 <= m.y, coh = u3 <= m.z;  // both cutoffs consumed on every path
         ++n_int;  // photoelectric, coherent or Compton
-        if (u3 <= m.y) {
+        if (photo) {
           ++n_photo;
           state = kDead;
         } else {
-          if (u3 <= m.z) {
+          if (coh) {
             ++n_coh;
             if (Jp->coherent) {
               const float cth = thomson_mu(next_u01(rs, Jp->job_id, Jp->key0, Jp->key1));
@@ -635,10 +640,16 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       const uint32_t vb = voxel_at(p, tb, *Jp, bx, by, bz);
       const int la = label_of(va, *Jp), lb = label_of(vb, *Jp);
       const float mua = mu_at(p, la, *Jp), mub = mu_at(p, lb, *Jp);
+      // Both attenuations are consumed here on every path (an empty asm taking
+      // them as inputs): a table load for a label above 3 left pending on the
+      // paths that end the flight made the compiler wait for every memory
+      // operation, tally atomics included, at the start of the next step.
+      asm volatile("" ::"v"(mua), "v"(mub));
+      const bool real_a = u01(c1) * p.mumax < mua, real_b = u01(c3) * p.mumax < mub;
       if (ta >= p.t1) {
         MONTE_NOTE_TALLY(tally(p, *Jp));
         state = kDead;
-      } else if (u01(c1) * p.mumax < mua) {  // real collision: park until the wave runs interactions
+      } else if (real_a) {  // real collision: park until the wave runs interactions
         p.px = ax; p.py = ay; p.pz = az;
         p.lab = la;
         state = kCollide;
@@ -647,7 +658,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         if (tb >= p.t1) {
           MONTE_NOTE_TALLY(tally(p, *Jp));
           state = kDead;
-        } else if (u01(c3) * p.mumax < mub) {
+        } else if (real_b) {
           p.px = bx; p.py = by; p.pz = bz;
           p.lab = lb;
           state = kCollide;
