@@ -531,6 +531,11 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           }
           const int iu = (int)u_, iv = (int)v_;
           const uint32_t pix = v_ * (uint32_t)Jp->nu + u_;
+          // the first flight's majorant comes from the labels its line crosses;
+          // loaded first, so the load is in flight during the spectrum draw
+          const uint32_t set = Jp->ray_set
+                                   ? (uint32_t)Jp->ray_set[(size_t)(unit_view - Jp->view_begin) * npix + pix]
+                                   : kAnyLabel;
           const float yp = ((float)iu + 0.5f - Jp->un0) * Jp->u_step;
           const float zp = ((float)iv + 0.5f - Jp->vn0) * Jp->v_step;
           const float in = rsqrtf(Jp->sdd * Jp->sdd + yp * yp + zp * zp);
@@ -552,10 +557,6 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           p.e = e;
           p.eb = ebin_of(e);
           p.nint = 0;
-          // the first flight's majorant comes from the labels its line crosses
-          const uint32_t set = Jp->ray_set
-                                   ? (uint32_t)Jp->ray_set[(size_t)(unit_view - Jp->view_begin) * npix + pix]
-                                   : kAnyLabel;
           if (start_flight(p, *Jp, wt, set)) {
             state = kFly;
           } else {
@@ -586,7 +587,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         const float4 m = Jp->tables->m[p.lab][p.eb];
         rs.left = 0;  // interactions start on a fresh block
         const float u3 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
-        const bool photo = u3 <= m.y, coh = u3 <= m.z;  // both cutoffs consumed on every path
+        asm volatile("" ::"v"(m.y), "v"(m.z));  // both cutoffs loaded and waited for together
+        const bool photo = u3 <= m.y, coh = u3 <= m.z;
         ++n_int;  // photoelectric, coherent or Compton
         if (photo) {
           ++n_photo;
