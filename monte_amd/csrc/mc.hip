@@ -9,8 +9,10 @@
 //     stream keyed by (seed) with counter (block, photon, view, job), so any
 //     history can be replayed and results do not depend on scheduling;
 //   * the kernel is persistent: waves pull batches of photons (work units)
-//     from a device queue shared by all jobs of a launch;
-//   * each loop iteration advances every flying lane by one Woodcock step.
+//     from device work queues shared by all jobs of a launch (32 counters,
+//     each block drawing from its own first);
+//   * each loop iteration advances every flying lane by up to two Woodcock
+//     steps (one Philox block).
 //     Lanes that collide (real interaction pending) or die (absorbed, capped,
 //     tallied) park; the wave runs the interaction code, or refills dead lanes
 //     from its batch (ballot + mbcnt prefix ranks), only once enough lanes
