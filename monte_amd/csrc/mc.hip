@@ -389,10 +389,10 @@ __device__ inline uint32_t voxel_at(const Photon& p, float t, const McJob& J, fl
   return mad_u24(mad_u24((uint32_t)iz, (uint32_t)J.ny, (uint32_t)iy), (uint32_t)J.nx, (uint32_t)ix);
 }
 
-// Label of a voxel, by a global-address-space load; labels past the table are
-// clamped so no table read leaves it.
-__device__ inline int label_of(uint32_t i, const McJob& J) {
-  return min((int)((const __attribute__((address_space(1))) uint8_t*)J.labels)[i], kMaxLabels - 1);
+// Label byte of a voxel, by a global-address-space load; the caller clamps
+// labels past the table (min(label, kMaxLabels - 1)) so no table read leaves it.
+__device__ inline uint32_t label_raw(uint32_t i, const McJob& J) {
+  return ((const __attribute__((address_space(1))) uint8_t*)J.labels)[i];
 }
 
 // A parked branch runs once at least this many lanes wait for it (or nothing
@@ -642,7 +642,15 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       float ax, ay, az, bx, by, bz;
       const uint32_t va = voxel_at(p, ta, *Jp, ax, ay, az);
       const uint32_t vb = voxel_at(p, tb, *Jp, bx, by, bz);
-      const int la = label_of(va, *Jp), lb = label_of(vb, *Jp);
+      // a step that leaves the flight box (most first flights end on their first
+      // step) reads no label: its load would only add traffic to the line misses.
+      // Both loads are issued before either value is used (the empty asm keeps
+      // the clamps, and so the waits, after both).
+      uint32_t ra = 0, rb = 0;
+      if (ta < p.t1) ra = label_raw(va, *Jp);
+      if (tb < p.t1) rb = label_raw(vb, *Jp);
+      asm volatile("" : "+v"(ra), "+v"(rb));
+      const int la = min((int)ra, kMaxLabels - 1), lb = min((int)rb, kMaxLabels - 1);
       const float mua = mu_at(p, la, *Jp), mub = mu_at(p, lb, *Jp);
       // Both attenuations are consumed here on every path (an empty asm taking
       // them as inputs): a table load for a label above 3 left pending on the
@@ -935,7 +943,7 @@ __global__ __launch_bounds__(256) void primary_exact_kernel(PrimaryGeom P, const
       if (sn[2] < sn[a]) a = 2;
       const double te = fmin(sn[a], t1);
       if (te > t) {
-        // labels past the table read as the last label, as in transport (label_of)
+        // labels past the table read as the last label, as in transport
         const int lab = min((int)labels[((size_t)idx[2] * P.n[1] + idx[1]) * P.n[0] + idx[0]], kMaxLabels - 1);
         lens[lab][tid] += te - t;
         t = te;
