@@ -15,9 +15,11 @@
 //     steps (one Philox block).
 //     Lanes that collide (real interaction pending) or die (absorbed, capped,
 //     tallied) park; the wave runs the interaction code, or refills dead lanes
-//     from its batch (ballot + mbcnt prefix ranks), only once enough lanes
-//     are parked or nothing else can run, so those branches execute near full
-//     width instead of for one lane at a time;
+//     (ballot + mbcnt prefix ranks), only once enough lanes are parked or
+//     nothing else can run, so those branches execute near full width instead
+//     of for one lane at a time;
+//   * photons are born 64 at a time, one per lane with every lane active, into
+//     a per-wave LDS buffer that dead lanes then take from;
 //   * flights are clipped to the bounding box of the voxels that attenuate
 //     (the reference clips to the phantom box, quirk Q9; mu is 0 in the
 //     margin between the two, so no history changes) and the first step
@@ -236,6 +238,17 @@ struct Photon {
 // Per-wave LDS copy of the current job's per-energy tables, read at every
 // birth and flight start (the spectrum search alone is 8 dependent reads).
 constexpr int kGuide = 256;  // spectrum guide cells
+
+// Photons born 64 at a time, one per lane with every lane active, wait here
+// until dead lanes take them (ray-box clip and majorant already done; all are
+// of one view).  Slots are the lanes that made them; slot[k] is the slot of
+// the k-th photon that entered the flight box.
+struct BirthBuffer {
+  float px[64], py[64], pz[64], dx[64], dy[64], dz[64], e[64], t1[64], mumax[64];
+  uint32_t id[64];
+  uint8_t slot[64];
+};
+
 struct WaveTables {
   float cdf[kRows];      // spectrum CDF (unused for mono jobs)
   float mumax[kRows];    // Woodcock majorant
@@ -243,6 +256,7 @@ struct WaveTables {
   // guide[g] = the energy row drawn for u = g / kGuide (rows are <= 200): a
   // draw u in [g, g + 1) / kGuide lies between guide[g] and guide[g + 1]
   uint8_t guide[kGuide + 4];
+  BirthBuffer bb;
 };
 
 // Inverse CDF of the spectrum (CBCT_real325.cu:479-486): the first row e in
@@ -414,6 +428,9 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   int unit_view = 0;
   long long pending = -1;
   bool exhausted = false;
+  uint32_t buf_head = 0, buf_n = 0;  // the birth buffer's next photon and photons left (wave-uniform)
+  int buf_view = 0;                  // the view of the buffered photons, and its cos and sin
+  float buf_cv = 0.f, buf_sv = 0.f;
 
   int state = kDead;
   Photon p{};
@@ -482,7 +499,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       }
       if (pending >= 0) {
         const int pj = __builtin_amdgcn_readfirstlane(job_of_unit(cjobs, n_jobs, (uint64_t)pending));
-        if (pj == job || all_dead) {
+        if (pj == job || (all_dead && buf_n == 0)) {  // buffered photons are of the current job
           const bool switched = pj != job;
           if (switched) {
             if (job >= 0) flush(counters + job);
@@ -511,63 +528,79 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         }
       }
     }
-    if (all_dead && next >= end && pending < 0 && exhausted) break;
+    if (all_dead && next >= end && pending < 0 && exhausted && buf_n == 0) break;
 
-    // 2) refill dead lanes from the current unit
-    if (dead != 0 && next < end && (popc(dead) >= kRefillBatch || fly == 0)) {
-      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
-      if (state == kDead) {
-        const uint32_t id = next + rank;
+    // 2) refill dead lanes from the birth buffer, after filling it when it is empty
+    if (dead != 0 && (buf_n != 0 || next < end) && (popc(dead) >= kRefillBatch || fly == 0)) {
+      if (buf_n == 0) {
+        // the unit's next 64 photons, one per lane with every lane active
+        const uint32_t id = next + (uint32_t)lane;
+        CU32* vcs = (CU32*)Jp->view_cs + 2 * unit_view;  // read-only during the launch: scalar loads
+        const float cv = __uint_as_float(vcs[0]), sv = __uint_as_float(vcs[1]);
+        bool born = false;
         if (id < end) {
           ++n_hist;
-          rs.ctr = 0; rs.photon = id; rs.view = (uint32_t)unit_view; rs.left = 0;
           // aim at the centre of pixel id mod (nu*nv) (CBCT_real325.cu:464-466), in the
           // order iv * nu + iu (u fastest, so a batch covers whole detector rows):
-          // stepped on from the wave's cursor, id - next = rank < 64 pixels on
+          // stepped on from the wave's cursor, id - next = lane < 64 pixels on
           const uint32_t npix = (uint32_t)(Jp->nu * Jp->nv);
-          uint32_t u_ = cur_iu + rank, v_ = cur_iv;
+          uint32_t u_ = cur_iu + (uint32_t)lane, v_ = cur_iv;
           while (u_ >= (uint32_t)Jp->nu) {
             u_ -= (uint32_t)Jp->nu;
             v_ = v_ + 1 == (uint32_t)Jp->nv ? 0u : v_ + 1;
           }
-          const int iu = (int)u_, iv = (int)v_;
           const uint32_t pix = v_ * (uint32_t)Jp->nu + u_;
           // the first flight's majorant comes from the labels its line crosses;
           // loaded first, so the load is in flight during the spectrum draw
           const uint32_t set = Jp->ray_set
                                    ? (uint32_t)Jp->ray_set[(size_t)(unit_view - Jp->view_begin) * npix + pix]
                                    : kAnyLabel;
-          const float yp = ((float)iu + 0.5f - Jp->un0) * Jp->u_step;
-          const float zp = ((float)iv + 0.5f - Jp->vn0) * Jp->v_step;
+          const float yp = ((float)u_ + 0.5f - Jp->un0) * Jp->u_step;
+          const float zp = ((float)v_ + 0.5f - Jp->vn0) * Jp->v_step;
           const float in = rsqrtf(Jp->sdd * Jp->sdd + yp * yp + zp * zp);
-          CU32* vcs = (CU32*)Jp->view_cs + 2 * unit_view;  // read-only during the launch: scalar loads
-          const float2 cs = make_float2(__uint_as_float(vcs[0]), __uint_as_float(vcs[1]));
-          p.view = unit_view;
-          p.cv = cs.x;
-          p.sv = cs.y;
-          p.dx = (Jp->sdd * cs.x - yp * cs.y) * in;
-          p.dy = (Jp->sdd * cs.y + yp * cs.x) * in;
-          p.dz = zp * in;
-          p.px = -Jp->sod * cs.x; p.py = -Jp->sod * cs.y; p.pz = 0.f;
+          Photon q;
+          q.view = unit_view;
+          q.cv = cv;
+          q.sv = sv;
+          // the fused products are spelled out, so the direction's rounding does not
+          // depend on which product the compiler would fuse here
+          q.dx = fmaf(Jp->sdd, cv, -(yp * sv)) * in;
+          q.dy = fmaf(cv, yp, Jp->sdd * sv) * in;
+          q.dz = zp * in;
+          q.px = -Jp->sod * cv; q.py = -Jp->sod * sv; q.pz = 0.f;
           float e = Jp->e_kev;
           if (Jp->spectrum_cdf) {
-            const float u = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
+            Stream bs{};
+            bs.photon = id; bs.view = (uint32_t)unit_view;
+            const float u = next_u01(bs, Jp->job_id, Jp->key0, Jp->key1);
             const int g = (int)(u * (float)kGuide);  // exact: u < 1 and kGuide is a power of 2
             e = (float)spectrum_row(wt.cdf, u, wt.guide[g], wt.guide[g + 1]);
           }
-          p.e = e;
-          p.eb = ebin_of(e);
-          p.nint = 0;
-          if (start_flight(p, *Jp, wt, set)) {
-            state = kFly;
+          q.e = e;
+          q.eb = ebin_of(e);
+          q.nint = 0;
+          born = start_flight(q, *Jp, wt, set);
+          if (born) {
+            wt.bb.px[lane] = q.px; wt.bb.py[lane] = q.py; wt.bb.pz[lane] = q.pz;
+            wt.bb.dx[lane] = q.dx; wt.bb.dy[lane] = q.dy; wt.bb.dz[lane] = q.dz;
+            wt.bb.e[lane] = q.e; wt.bb.t1[lane] = q.t1; wt.bb.mumax[lane] = q.mumax;
+            wt.bb.id[lane] = id;
           } else {
-            MONTE_NOTE_TALLY(tally(p, *Jp));
+            MONTE_NOTE_TALLY(tally(q, *Jp));
           }
         }
-      }
-      {
-        const uint32_t nxt = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(end, next + popc(dead)));
+        const uint64_t bm = __ballot(born);
+        if (born)
+          wt.bb.slot[__builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] =
+              (uint8_t)lane;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the buffer in LDS is complete
+        __builtin_amdgcn_wave_barrier();
+        buf_n = (uint32_t)popc(bm);
+        buf_head = 0;
+        buf_view = unit_view;
+        buf_cv = cv;
+        buf_sv = sv;
+        const uint32_t nxt = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(end, next + 64u));
         uint32_t u_ = cur_iu + (nxt - next), v_ = cur_iv;
         while (u_ >= (uint32_t)Jp->nu) {
           u_ -= (uint32_t)Jp->nu;
@@ -577,8 +610,37 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         cur_iv = (uint32_t)__builtin_amdgcn_readfirstlane((int)v_);
         next = nxt;
       }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
+      if (state == kDead && rank < buf_n) {
+        const int s = wt.bb.slot[buf_head + rank];
+        p.px = wt.bb.px[s]; p.py = wt.bb.py[s]; p.pz = wt.bb.pz[s];
+        p.dx = wt.bb.dx[s]; p.dy = wt.bb.dy[s]; p.dz = wt.bb.dz[s];
+        p.e = wt.bb.e[s];
+        p.eb = ebin_of(p.e);
+        p.t = 0.f;
+        p.t1 = wt.bb.t1[s];
+        p.mumax = wt.bb.mumax[s];
+        p.lscale = 0.69314718055994531f * rcp(p.mumax);  // as start_flight forms it
+        p.mu0 = wt.mu[0][p.eb];
+        p.mu1 = wt.mu[1][p.eb];
+        p.mu2 = wt.mu[2][p.eb];
+        p.mu3 = wt.mu[3][p.eb];
+        p.nint = 0;
+        p.view = buf_view;
+        p.cv = buf_cv;
+        p.sv = buf_sv;
+        // the spectrum draw took block 0 of the photon's stream; flights take whole blocks
+        rs.ctr = Jp->spectrum_cdf ? 1u : 0u;
+        rs.photon = wt.bb.id[s];
+        rs.view = (uint32_t)buf_view;
+        rs.left = 0;
+        state = kFly;
+      }
+      const uint32_t took = min((uint32_t)popc(dead), buf_n);
+      buf_head += took;
+      buf_n -= took;
     }
-
     // 3) interactions of the parked colliding lanes
     if (coll != 0 && (popc(coll) >= kInteractBatch || fly == 0)) {
       if (state == kCollide && p.nint >= Jp->max_int) {
