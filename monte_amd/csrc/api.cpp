@@ -280,8 +280,8 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
     return fail(c, MONTE_ERR_UNSUPPORTED, "phantom ny * nz and nx must be below 2^24 (24-bit voxel indexing)");
   if (ph->n_labels < 1 || ph->n_labels > kMaxLabels) return fail(c, MONTE_ERR_INVALID, "n_labels out of range");
   if (c->n_tables < 1) return fail(c, MONTE_ERR_STATE, "attenuation tables not loaded (monte_load_tables)");
-  // photon indices are 32-bit: a unit's end (next + batch) and the refill
-  // cursor (next + 64) must not wrap
+  // photon indices are 32-bit (the kernel forms cycle * nu nv + pixel in 64 bits
+  // and keeps the photons below photons_per_view)
   if (cfg->photons_per_view == 0 || cfg->photons_per_view > 0x100000000ull - (uint64_t)batch - 64)
     return fail(c, MONTE_ERR_INVALID, "photons_per_view must be in [1, 2^32 - batch - 64] (batch = 64 * mc_batch)");
   if (cfg->max_interactions < 0) return fail(c, MONTE_ERR_INVALID, "max_interactions < 0");
@@ -386,7 +386,23 @@ int build_job(monte_ctx* c, const monte_geom* g, const monte_phantom* ph, const 
   J.key1 = (uint32_t)(cfg->seed >> 32);
   J.job_id = cfg->job;
   J.photons_per_view = (uint32_t)cfg->photons_per_view;
-  J.batches_per_view = (uint32_t)((cfg->photons_per_view + batch - 1) / batch);
+  {
+    // work units: groups of detector tiles over blocks of photon cycles, about
+    // `batch` photons each (tiles are grouped when a view has fewer than batch / 64
+    // cycles); only tile rows that hold photons count when a view has fewer
+    // photons than pixels
+    const uint64_t npix = (uint64_t)J.nu * (uint64_t)J.nv, ppv = cfg->photons_per_view;
+    const uint64_t cycles = (ppv + npix - 1) / npix;
+    const uint64_t rows = (std::min(ppv, npix) + J.nu - 1) / J.nu;
+    const uint64_t per_unit = std::max<uint64_t>(1, (uint64_t)batch / 64);
+    J.tiles_u = (uint32_t)((J.nu + kTileU - 1) / kTileU);
+    J.tiles = (uint32_t)(J.tiles_u * ((rows + kTileV - 1) / kTileV));
+    J.cycles = (uint32_t)cycles;
+    J.cycle_block = (uint32_t)std::min(per_unit, cycles);
+    J.tile_group = (uint32_t)std::max<uint64_t>(1, per_unit / J.cycle_block);
+    J.cycle_blocks = (uint32_t)((cycles + J.cycle_block - 1) / J.cycle_block);
+    J.batches_per_view = ((J.tiles + J.tile_group - 1) / J.tile_group) * J.cycle_blocks;
+  }
   J.unit_count = (uint64_t)J.batches_per_view * (uint64_t)J.n_views_sim;
   return MONTE_OK;
 }
@@ -549,14 +565,13 @@ int prepare_mc(monte_ctx* c, std::vector<HostJob>& hj, int block, int blocks_per
 
 // Resets the work queue and counters, then launches the transport (with
 // `start` recorded between the two when given, so it times the transport alone).
-hipError_t launch_prepared(monte_ctx* c, const McPrepared& P, int batch, int block, int blocks_per_cu,
+hipError_t launch_prepared(monte_ctx* c, const McPrepared& P, int block, int blocks_per_cu,
                            hipEvent_t start = nullptr) {
   hipError_t e = launch_mc_reset(P.queue(), P.counters(), P.n, c->stream);
   if (e != hipSuccess) return e;
   if (start && (e = hipEventRecord(start, c->stream)) != hipSuccess) return e;
   const int grid = std::max(1, c->n_cu * blocks_per_cu);
   return launch_mc_transport((const McJob*)(P.base + P.L.jobs_off), P.n, P.units, P.chunk, P.queue(), P.counters(),
-                             batch,
                              block, grid, c->stream);
 }
 
@@ -569,12 +584,12 @@ int read_counters(monte_ctx* c, const McPrepared& P, std::vector<McCounters>& cn
 
 // Prepares and runs one launch in a device buffer: the context's MC workspace,
 // or `base` (at least mc_layout(hj).total bytes) when the caller provides one.
-int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int batch, int block, int blocks_per_cu, float* ms_out,
+int run_mc(monte_ctx* c, std::vector<HostJob>& hj, int block, int blocks_per_cu, float* ms_out,
            std::vector<McCounters>* counters_out, char* base = nullptr, int ray_slot = kSlotRay) {
   McPrepared P;
   int rc = prepare_mc(c, hj, block, blocks_per_cu, base, P, ray_slot);
   if (rc) return rc;
-  const hipError_t le = launch_prepared(c, P, batch, block, blocks_per_cu, c->ev0);
+  const hipError_t le = launch_prepared(c, P, block, blocks_per_cu, c->ev0);
   MONTE_HIP(c, hipEventRecord(c->ev1, c->stream));
   std::vector<McCounters> cnt;
   if ((rc = read_counters(c, P, cnt))) return rc;
@@ -624,7 +639,7 @@ int e2e_launches(monte_ctx* c, const monte_geom& g, const monte_e2e_buffers& b, 
   MONTE_HIP(c, launch_fill((float*)b.primary, 0.f, 2 * np, c->stream));
   MONTE_HIP(c, launch_fill((float*)b.scatter, 0.f, 2 * np, c->stream));
   const monte_tuning& t = c->tuning;
-  MONTE_HIP(c, launch_prepared(c, c->e2e_mc, 64 * t.mc_batch, t.mc_block, t.mc_blocks_per_cu));
+  MONTE_HIP(c, launch_prepared(c, c->e2e_mc, t.mc_block, t.mc_blocks_per_cu));
   MONTE_HIP(c, launch_counts_to_lineint(b.primary, b.scatter, b.lineint, g.n_views, g.nu, g.nv, ppv, c->stream));
   float* pad = (float*)c->ws[kSlotFwd];
   int rc = run_filter_padded(c, g, b.lineint, pad);
@@ -954,7 +969,7 @@ int monte_mc_batch(monte_ctx* c, const monte_mc_job* jobs, int n_jobs, monte_mc_
   }
   float ms = 0.f;
   std::vector<McCounters> cnt;
-  int rc = run_mc(c, hj, batch, c->tuning.mc_block, c->tuning.mc_blocks_per_cu, &ms, &cnt);
+  int rc = run_mc(c, hj, c->tuning.mc_block, c->tuning.mc_blocks_per_cu, &ms, &cnt);
   if (rc) return rc;
   if (stats)
     for (int i = 0; i < n_jobs; ++i) fill_stats(&stats[i], cnt[i], ms);
@@ -1003,7 +1018,7 @@ int monte_mc_project_noalloc(monte_ctx* c, const monte_geom* g, const monte_phan
   hj[0].d.scatter = scatter;
   float ms = 0.f;
   std::vector<McCounters> cnt;
-  if ((rc = run_mc(c, hj, batch, c->tuning.mc_block, c->tuning.mc_blocks_per_cu, &ms, &cnt, scratch, -1))) return rc;
+  if ((rc = run_mc(c, hj, c->tuning.mc_block, c->tuning.mc_blocks_per_cu, &ms, &cnt, scratch, -1))) return rc;
   if (stats) fill_stats(stats, cnt[0], ms);
   return MONTE_OK;
 }
@@ -1223,7 +1238,7 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
           hj.d.scatter = tally + np;
           std::vector<HostJob> v(1, hj);
           float ms = 0.f;
-          if ((rc = run_mc(c, v, 64 * bt, blk, bpc, &ms, nullptr))) return rc;
+          if ((rc = run_mc(c, v, blk, bpc, &ms, nullptr))) return rc;
           if (ms < best_ms) {
             best_ms = ms;
             best.mc_block = blk;

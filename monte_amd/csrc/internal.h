@@ -31,6 +31,8 @@ __host__ __device__ inline double xmul(double a, double b) {
 
 constexpr int kRows = MONTE_TABLE_ROWS;
 constexpr int kMaxLabels = MONTE_MAX_LABELS;
+// MC detector tile: one birth of a wave (64 lanes) covers one tile of one cycle.
+constexpr int kTileU = 8, kTileV = 8;
 constexpr int kPadLo = 1;  // zero guard cells before u / v in padded filtered projections
 constexpr int kPadHi = 3;  // and after (covers the +1 bilinear tap at the window edge)
 
@@ -123,7 +125,15 @@ struct McJob {
   int max_int, coherent;
   uint32_t key0, key1, job_id;
   uint32_t photons_per_view;
-  uint32_t batches_per_view;     // ceil(photons_per_view / batch)
+  // Photon n of a view aims at pixel j = n mod (nu nv) in aim order (iv nu + iu),
+  // in cycle c = n / (nu nv).  A work unit is a group of `tile_group` detector
+  // tiles of kTileU x kTileV pixels (tiles numbered u fastest, `tiles_u` per row,
+  // `tiles` in all) over `cycle_block` consecutive cycles, and a birth takes one
+  // tile of one cycle: so a unit's primary tallies fall on a few lines that it
+  // hits once per cycle, and the units in flight share a narrow detector band.
+  uint32_t tiles_u, tiles, tile_group;
+  uint32_t cycles, cycle_block, cycle_blocks;  // ceil(photons / (nu nv)); per unit; blocks
+  uint32_t batches_per_view;     // ceil(tiles / tile_group) * cycle_blocks
   uint64_t unit_begin;           // first global work unit of this job
   uint64_t unit_count;
   const uint8_t* ray_set;        // [n_views_sim][nu nv] first-flight label sets (aim order), or null
@@ -302,8 +312,7 @@ constexpr int kMcQueues = 32;
 constexpr int kMcQueueStride = 16;
 constexpr size_t kMcQueueBytes = sizeof(unsigned int) * kMcQueues * kMcQueueStride;
 hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, uint32_t chunk,
-                               unsigned int* queue, McCounters* counters, int batch, int block, int grid,
-                               hipStream_t s);
+                               unsigned int* queue, McCounters* counters, int block, int grid, hipStream_t s);
 hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs, hipStream_t s);
 // First-flight label sets: per brick of b^3 voxels widened by one voxel
 // (bricks [nb2][nb1][nb0]), then per view of [view_begin, view_begin +

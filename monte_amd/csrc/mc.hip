@@ -419,12 +419,14 @@ constexpr unsigned kRefillBatch = 16;
 __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restrict__ jobs, int n_jobs,
                                                            uint64_t total_units, uint32_t chunk,
                                                            unsigned int* __restrict__ queue,
-                                                           McCounters* __restrict__ counters, int batch) {
+                                                           McCounters* __restrict__ counters) {
   int job = -1;
   const int q_first = (int)(blockIdx.x % kMcQueues);
   int q_tried = 0;  // queues found spent (the wave then draws from the next)
-  uint32_t next = 0, end = 0;  // photon range of the current unit (wave-uniform)
-  uint32_t cur_iu = 0, cur_iv = 0;  // pixel photon `next` aims at (wave-uniform), so births need no division
+  // the current unit (wave-uniform): tiles [unit_t0, unit_t0 + unit_nt) over photon
+  // cycles [unit_cyc, unit_cyc_end); the next birth takes tile unit_t0 + unit_g of
+  // cycle unit_cyc
+  uint32_t unit_t0 = 0, unit_nt = 0, unit_g = 0, unit_cyc = 0, unit_cyc_end = 0;
   int unit_view = 0;
   long long pending = -1;
   bool exhausted = false;
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
     const bool all_dead = (fly | coll) == 0;
 
     // 1) secure a work unit when lanes are idle and the current one is spent
-    if (dead != 0 && next >= end) {
+    if (dead != 0 && unit_cyc >= unit_cyc_end) {
       if (pending < 0 && !exhausted) {
         // in round r = k / chunk, queue q hands out chunk r kMcQueues + (q + r) mod kMcQueues,
         // unit k mod chunk of it: the chunks of a round are dealt to the queues rotated by
@@ -505,19 +507,23 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             if (job >= 0) flush(counters + job);
             job = pj;
           }
-          // units run views fastest: the units in flight share photon batches, hence
-          // detector rows, hence a thin z-slab of the phantom that stays in L2
+          // units run views fastest, then cycle blocks, then tile groups: the units in
+          // flight share a few detector tiles, hence a thin z-slab of the phantom that
+          // stays in L2, and each unit's primary tallies stay on a few lines
           const uint64_t local = (uint64_t)pending - cjobs[pj].unit_begin;
           const uint32_t nvs = (uint32_t)cjobs[pj].n_views_sim;
           const uint32_t bi = (uint32_t)(local / nvs);
           const uint32_t vl = (uint32_t)(local - (uint64_t)bi * nvs);
           unit_view = __builtin_amdgcn_readfirstlane(cjobs[pj].view_begin + (int)vl);
-          next = (uint32_t)__builtin_amdgcn_readfirstlane((int)(bi * (uint32_t)batch));
-          end = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(next + (uint32_t)batch, cjobs[pj].photons_per_view));
           {
-            const uint32_t nu_j = (uint32_t)cjobs[pj].nu, pix0 = next % (nu_j * (uint32_t)cjobs[pj].nv);
-            cur_iv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(pix0 / nu_j));
-            cur_iu = (uint32_t)__builtin_amdgcn_readfirstlane((int)(pix0 - cur_iv * nu_j));
+            const uint32_t ncb = cjobs[pj].cycle_blocks, grp = cjobs[pj].tile_group;
+            const uint32_t tg = bi / ncb, cb = bi - tg * ncb;
+            unit_t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tg * grp));
+            unit_nt = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(grp, cjobs[pj].tiles - tg * grp));
+            unit_g = 0;
+            unit_cyc = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cb * cjobs[pj].cycle_block));
+            unit_cyc_end = (uint32_t)__builtin_amdgcn_readfirstlane(
+                (int)min(unit_cyc + cjobs[pj].cycle_block, cjobs[pj].cycles));
           }
           pending = -1;
           if (switched) {  // J is re-read for the new job at the top of the loop
@@ -528,28 +534,27 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         }
       }
     }
-    if (all_dead && next >= end && pending < 0 && exhausted && buf_n == 0) break;
+    if (all_dead && unit_cyc >= unit_cyc_end && pending < 0 && exhausted && buf_n == 0) break;
 
     // 2) refill dead lanes from the birth buffer, after filling it when it is empty
-    if (dead != 0 && (buf_n != 0 || next < end) && (popc(dead) >= kRefillBatch || fly == 0)) {
+    if (dead != 0 && (buf_n != 0 || unit_cyc < unit_cyc_end) && (popc(dead) >= kRefillBatch || fly == 0)) {
       if (buf_n == 0) {
-        // the unit's next 64 photons, one per lane with every lane active
-        const uint32_t id = next + (uint32_t)lane;
+        // one tile of the unit in one cycle, one photon per lane with every lane
+        // active: lane l aims at the centre of pixel (u_, v_) of the tile
+        // (CBCT_real325.cu:464-466), and is photon n = cycle * nu nv + v_ nu + u_ of
+        // the view, whose pixel in aim order is n mod (nu nv)
+        const uint32_t t = unit_t0 + unit_g;
+        const uint32_t tv = t / (uint32_t)Jp->tiles_u, tu = t - tv * (uint32_t)Jp->tiles_u;
+        const uint32_t u_ = tu * kTileU + (uint32_t)lane % kTileU, v_ = tv * kTileV + (uint32_t)lane / kTileU;
+        const uint32_t npix = (uint32_t)(Jp->nu * Jp->nv);
+        const uint32_t pix = v_ * (uint32_t)Jp->nu + u_;
+        const uint64_t n64 = (uint64_t)unit_cyc * npix + pix;
+        const uint32_t id = (uint32_t)n64;
         CU32* vcs = (CU32*)Jp->view_cs + 2 * unit_view;  // read-only during the launch: scalar loads
         const float cv = __uint_as_float(vcs[0]), sv = __uint_as_float(vcs[1]);
         bool born = false;
-        if (id < end) {
+        if (u_ < (uint32_t)Jp->nu && v_ < (uint32_t)Jp->nv && n64 < Jp->photons_per_view) {
           ++n_hist;
-          // aim at the centre of pixel id mod (nu*nv) (CBCT_real325.cu:464-466), in the
-          // order iv * nu + iu (u fastest, so a batch covers whole detector rows):
-          // stepped on from the wave's cursor, id - next = lane < 64 pixels on
-          const uint32_t npix = (uint32_t)(Jp->nu * Jp->nv);
-          uint32_t u_ = cur_iu + (uint32_t)lane, v_ = cur_iv;
-          while (u_ >= (uint32_t)Jp->nu) {
-            u_ -= (uint32_t)Jp->nu;
-            v_ = v_ + 1 == (uint32_t)Jp->nv ? 0u : v_ + 1;
-          }
-          const uint32_t pix = v_ * (uint32_t)Jp->nu + u_;
           // the first flight's majorant comes from the labels its line crosses;
           // loaded first, so the load is in flight during the spectrum draw
           const uint32_t set = Jp->ray_set
@@ -600,15 +605,10 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         buf_view = unit_view;
         buf_cv = cv;
         buf_sv = sv;
-        const uint32_t nxt = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(end, next + 64u));
-        uint32_t u_ = cur_iu + (nxt - next), v_ = cur_iv;
-        while (u_ >= (uint32_t)Jp->nu) {
-          u_ -= (uint32_t)Jp->nu;
-          v_ = v_ + 1 == (uint32_t)Jp->nv ? 0u : v_ + 1;
+        if (++unit_g == unit_nt) {  // the unit's tiles are done for this cycle
+          unit_g = 0;
+          ++unit_cyc;
         }
-        cur_iu = (uint32_t)__builtin_amdgcn_readfirstlane((int)u_);
-        cur_iv = (uint32_t)__builtin_amdgcn_readfirstlane((int)v_);
-        next = nxt;
       }
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
@@ -926,11 +926,10 @@ hipError_t launch_mc_reset(unsigned int* queue, McCounters* counters, int n_jobs
 }
 
 hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, uint32_t chunk,
-                               unsigned int* queue, McCounters* counters, int batch, int block, int grid,
-                               hipStream_t s) {
+                               unsigned int* queue, McCounters* counters, int block, int grid, hipStream_t s) {
   if (block % 64 || block > 256 || block <= 0 || grid <= 0 || chunk == 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(mc_transport_kernel, dim3(grid), dim3(block), sizeof(WaveTables) * (block / 64), s, jobs_dev,
-                     n_jobs, total_units, chunk, queue, counters, batch);
+                     n_jobs, total_units, chunk, queue, counters);
   return hipGetLastError();
 }
 
