@@ -249,7 +249,9 @@ struct BirthBuffer {
   uint8_t slot[64];
 };
 
+constexpr int kWaveStats = 10;
 struct WaveTables {
+  unsigned long long stats[kWaveStats];  // the wave's event counts for the current job
   float cdf[kRows];      // spectrum CDF (unused for mono jobs)
   float mumax[kRows];    // Woodcock majorant
   float mu[4][kRows];    // mu of labels 0..3
@@ -270,24 +272,37 @@ __device__ inline int spectrum_row(const float* cdf, float u, int lo, int hi) {
 }
 
 enum { kDead = 0, kFly = 1, kCollide = 2 };
-// Per-lane statistics, counted in the branches where the events happen and
-// summed over the wave when a job's counters are flushed.  (Kept as separate
-// scalars: as one struct the compiler left them in scratch memory.)
+// Statistics are counted per wave, not per lane: where every lane of the wave
+// is active, a ballot of the lanes where an event happened is added to the
+// wave's 64-bit counter in LDS, and the counters go to the job's when it is
+// flushed.  (Ten per-lane counters, incremented in the branches, took 56 of the
+// kernel's 121 VGPRs.)
 enum { kStatHistories, kStatSteps, kStatInteractions, kStatPhoto, kStatCoherent, kStatCompton, kStatTp, kStatTs,
        kStatMissed, kStatCapped, kStats };
+static_assert(kStats == kWaveStats, "one LDS counter per statistic");
 
 __device__ inline unsigned popc(uint64_t m) { return (unsigned)__popcll(m); }
 
-// Sum over the wave in 64 bits: 64 lanes of 32-bit counts can exceed 2^32.
-__device__ inline unsigned long long wave_sum(uint32_t v) {
-  unsigned long long s = v;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  return s;
+// Adds the lanes where `pred` holds to the wave's counter k.  Called where
+// every lane of the wave is active; lane 0 adds for the wave.
+__device__ inline void count_lanes(WaveTables& wt, int k, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if (__lane_id() == 0 && m != 0) atomicAdd(&wt.stats[k], (unsigned long long)popc(m));
 }
 
 // Straight line from the current position to the detector plane x' = sdd - sod.
 enum { kTallyMiss = 0, kTallyPrimary = 1, kTallyScatter = 2 };
+
+// Counts the tallies of a section from each lane's tally code (-1: none).
+__device__ inline void count_tallies(WaveTables& wt, int code) {
+  const uint64_t mp = __ballot(code == kTallyPrimary), ms = __ballot(code == kTallyScatter),
+                 mm = __ballot(code == kTallyMiss);
+  if (__lane_id() == 0 && (mp | ms | mm) != 0) {
+    atomicAdd(&wt.stats[kStatTp], (unsigned long long)popc(mp));
+    atomicAdd(&wt.stats[kStatTs], (unsigned long long)popc(ms));
+    atomicAdd(&wt.stats[kStatMissed], (unsigned long long)popc(mm));
+  }
+}
 
 __device__ inline int tally(const Photon& p, const McJob& J) {
   const float cx = p.cv, cy = p.sv;
@@ -437,36 +452,29 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   int state = kDead;
   Photon p{};
   Stream rs{};
-  // (counted by value: counters handed to tally() by reference were left in
-  // scratch memory by the compiler)
-#define MONTE_NOTE_TALLY(expr)               \
-  do {                                       \
-    const int code_ = (expr);                \
-    n_tp += code_ == kTallyPrimary ? 1u : 0u; \
-    n_ts += code_ == kTallyScatter ? 1u : 0u; \
-    n_miss += code_ == kTallyMiss ? 1u : 0u;  \
-  } while (0)
-  uint32_t n_hist = 0, n_steps = 0, n_int = 0, n_photo = 0, n_coh = 0, n_comp = 0, n_tp = 0, n_ts = 0, n_miss = 0,
-           n_cap = 0;
-  auto flush = [&](McCounters* dst) {
-    const unsigned long long v[kStats] = {wave_sum(n_hist), wave_sum(n_steps), wave_sum(n_int), wave_sum(n_photo),
-                                          wave_sum(n_coh),  wave_sum(n_comp),  wave_sum(n_tp),  wave_sum(n_ts),
-                                          wave_sum(n_miss), wave_sum(n_cap)};
-    if (__lane_id() == 0) {
-      atomicAdd(&dst->histories, v[kStatHistories]); atomicAdd(&dst->steps, v[kStatSteps]);
-      atomicAdd(&dst->interactions, v[kStatInteractions]); atomicAdd(&dst->photo, v[kStatPhoto]);
-      atomicAdd(&dst->coherent, v[kStatCoherent]); atomicAdd(&dst->compton, v[kStatCompton]);
-      atomicAdd(&dst->tallied_primary, v[kStatTp]); atomicAdd(&dst->tallied_scatter, v[kStatTs]);
-      atomicAdd(&dst->missed, v[kStatMissed]); atomicAdd(&dst->capped, v[kStatCapped]);
-    }
-    n_hist = n_steps = n_int = n_photo = n_coh = n_comp = n_tp = n_ts = n_miss = n_cap = 0;
-  };
   const int lane = __lane_id();
   // The job descriptors do not change during the launch: read them through the
   // constant address space, so wave-uniform fields come in by scalar loads.
   const CJob* cjobs = (const CJob*)jobs;
   extern __shared__ WaveTables wave_tables[];  // one slot per wave of the block
   WaveTables& wt = wave_tables[threadIdx.x >> 6];
+  if (lane < kStats) wt.stats[lane] = 0;
+  // adds the wave's counters to the job's and clears them
+  auto flush = [&](McCounters* dst) {
+    if (__lane_id() == 0) {
+      unsigned long long v[kStats];
+#pragma unroll
+      for (int k = 0; k < kStats; ++k) {
+        v[k] = wt.stats[k];
+        wt.stats[k] = 0;
+      }
+      atomicAdd(&dst->histories, v[kStatHistories]); atomicAdd(&dst->steps, v[kStatSteps]);
+      atomicAdd(&dst->interactions, v[kStatInteractions]); atomicAdd(&dst->photo, v[kStatPhoto]);
+      atomicAdd(&dst->coherent, v[kStatCoherent]); atomicAdd(&dst->compton, v[kStatCompton]);
+      atomicAdd(&dst->tallied_primary, v[kStatTp]); atomicAdd(&dst->tallied_scatter, v[kStatTs]);
+      atomicAdd(&dst->missed, v[kStatMissed]); atomicAdd(&dst->capped, v[kStatCapped]);
+    }
+  };
 
   for (;;) {
     const McJob J = load_job(cjobs, __builtin_amdgcn_readfirstlane(job < 0 ? 0 : job));
@@ -553,8 +561,9 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         CU32* vcs = (CU32*)Jp->view_cs + 2 * unit_view;  // read-only during the launch: scalar loads
         const float cv = __uint_as_float(vcs[0]), sv = __uint_as_float(vcs[1]);
         bool born = false;
-        if (u_ < (uint32_t)Jp->nu && v_ < (uint32_t)Jp->nv && n64 < Jp->photons_per_view) {
-          ++n_hist;
+        int bcode = -1;  // tally code of a photon that misses the flight box
+        const bool valid = u_ < (uint32_t)Jp->nu && v_ < (uint32_t)Jp->nv && n64 < Jp->photons_per_view;
+        if (valid) {
           // the first flight's majorant comes from the labels its line crosses;
           // loaded first, so the load is in flight during the spectrum draw
           const uint32_t set = Jp->ray_set
@@ -591,9 +600,11 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             wt.bb.e[lane] = q.e; wt.bb.t1[lane] = q.t1; wt.bb.mumax[lane] = q.mumax;
             wt.bb.id[lane] = id;
           } else {
-            MONTE_NOTE_TALLY(tally(q, *Jp));
+            bcode = tally(q, *Jp);
           }
         }
+        count_lanes(wt, kStatHistories, valid);
+        count_tallies(wt, bcode);
         const uint64_t bm = __ballot(born);
         if (born)
           wt.bb.slot[__builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] =
@@ -643,30 +654,30 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
     }
     // 3) interactions of the parked colliding lanes
     if (coll != 0 && (popc(coll) >= kInteractBatch || fly == 0)) {
-      if (state == kCollide && p.nint >= Jp->max_int) {
-        ++n_cap;
-        state = kDead;
-      }
+      const bool cap = state == kCollide && p.nint >= Jp->max_int;
+      count_lanes(wt, kStatCapped, cap);
+      if (cap) state = kDead;
+      int ev = 0;      // 1 photoelectric, 2 coherent, 3 Compton
+      int icode = -1;  // tally code of a scattered photon that leaves the flight box
       if (state == kCollide) {
         const float4 m = Jp->tables->m[p.lab][p.eb];
         rs.left = 0;  // interactions start on a fresh block
         const float u3 = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);
         asm volatile("" ::"v"(m.y), "v"(m.z));  // both cutoffs loaded and waited for together
         const bool photo = u3 <= m.y, coh = u3 <= m.z;
-        ++n_int;  // photoelectric, coherent or Compton
         if (photo) {
-          ++n_photo;
+          ev = 1;
           state = kDead;
         } else {
           if (coh) {
-            ++n_coh;
+            ev = 2;
             if (Jp->coherent) {
               const float cth = thomson_mu(next_u01(rs, Jp->job_id, Jp->key0, Jp->key1));
               const float w = next_u01(rs, Jp->job_id, Jp->key0, Jp->key1);  // azimuth / 2 pi
               rotate_dir(p.dx, p.dy, p.dz, cth, w);
             }
           } else {
-            ++n_comp;
+            ev = 3;
             const float lam = 511.0f * rcp(p.e);
             const float ro = kahn(lam, rs, Jp->job_id, Jp->key0, Jp->key1);
             const float lamd = ro * lam;
@@ -681,19 +692,26 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           if (start_flight(p, *Jp, wt, kAnyLabel)) {
             state = kFly;
           } else {
-            MONTE_NOTE_TALLY(tally(p, *Jp));
+            icode = tally(p, *Jp);
             state = kDead;
           }
         }
       }
+      count_lanes(wt, kStatInteractions, ev != 0);
+      count_lanes(wt, kStatPhoto, ev == 1);
+      count_lanes(wt, kStatCoherent, ev == 2);
+      count_lanes(wt, kStatCompton, ev == 3);
+      count_tallies(wt, icode);
     }
 
     // 4) two Woodcock steps for every flying lane from one Philox block
     //    (distance, accept, distance, accept); both label loads are issued
     //    before either is used, and a flight ending on the first step leaves
     //    the block's other two uniforms unused.
+    count_lanes(wt, kStatSteps, state == kFly);
+    int scode = -1;     // tally code of a flight that ends
+    bool two = false;   // took the second step of the pair
     if (state == kFly) {
-      ++n_steps;
       uint32_t c0 = rs.ctr, c1 = rs.photon, c2 = rs.view, c3 = Jp->job_id;
       philox10(c0, c1, c2, c3, Jp->key0, Jp->key1);
       rs.ctr += 1;
@@ -721,16 +739,16 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       asm volatile("" ::"v"(mua), "v"(mub));
       const bool real_a = u01(c1) * p.mumax < mua, real_b = u01(c3) * p.mumax < mub;
       if (ta >= p.t1) {
-        MONTE_NOTE_TALLY(tally(p, *Jp));
+        scode = tally(p, *Jp);
         state = kDead;
       } else if (real_a) {  // real collision: park until the wave runs interactions
         p.px = ax; p.py = ay; p.pz = az;
         p.lab = la;
         state = kCollide;
       } else {
-        ++n_steps;
+        two = true;
         if (tb >= p.t1) {
-          MONTE_NOTE_TALLY(tally(p, *Jp));
+          scode = tally(p, *Jp);
           state = kDead;
         } else if (real_b) {
           p.px = bx; p.py = by; p.pz = bz;
@@ -741,9 +759,10 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         }
       }
     }
+    count_lanes(wt, kStatSteps, two);
+    count_tallies(wt, scode);
   }
   if (job >= 0) flush(counters + job);
-#undef MONTE_NOTE_TALLY
 }
 
 // --------------------------------------------- first-flight label sets
