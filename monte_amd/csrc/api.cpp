@@ -654,14 +654,14 @@ int e2e_launches(monte_ctx* c, const monte_geom& g, const monte_e2e_buffers& b, 
 monte_tuning default_tuning() {
   monte_tuning t{};
   t.mc_block = 256;
-  // 5 blocks of 256 per CU is the occupancy of the transport kernel (5 waves per
-  // SIMD: its per-wave LDS slot allows 5, its 76 VGPRs 6).  A unit of 64 cycles of
+  // 6 blocks of 256 per CU is the occupancy of the transport kernel (6 waves per
+  // SIMD: its 6.6 KB LDS slot per wave and its 77 VGPRs both allow 6).  A unit of 64 cycles of
   // an 8 x 8 detector tile was fastest at 325^3 / 512^2 / 360 views x 1e7 photons
   // (102.1 ms at 4 waves per SIMD; 102.6, 103.7 and 108.9 ms for 32, 128 and 256
   // cycles): fewer, larger units contend less on the queue counters, smaller ones
   // keep each XCD's units on fewer detector tiles.
   t.mc_batch = 64;
-  t.mc_blocks_per_cu = 5;
+  t.mc_blocks_per_cu = 6;
   t.bp_zchunk = 16;
   t.bp_tile_x = 8;   // fastest at 325^3 / 512^2 / 360 views
   t.bp_tile_y = 32;
@@ -1226,7 +1226,7 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
     float best_ms = 1e30f;
     const int blocks[] = {64, 128, 256};
     const int batches[] = {32, 64, 128};
-    const int waves_per_cu[] = {12, 16, 20};  // 20 is the kernel's occupancy (5 waves per SIMD)
+    const int waves_per_cu[] = {16, 20, 24};  // 24 is the kernel's occupancy (6 waves per SIMD)
     for (int blk : blocks)
       for (int bt : batches)
         for (int wpc : waves_per_cu) {

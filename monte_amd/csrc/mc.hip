@@ -240,13 +240,16 @@ struct Photon {
 constexpr int kGuide = 256;  // spectrum guide cells
 
 // Photons born 64 at a time, one per lane with every lane active, wait here
-// until dead lanes take them (ray-box clip and majorant already done; all are
-// of one view).  Slots are the lanes that made them; slot[k] is the slot of
-// the k-th photon that entered the flight box.
+// until dead lanes take them (ray-box clip and label set already done; all are
+// of one view and one cycle of one detector tile).  Slots are the lanes that
+// made them, i.e. the tile's pixels; slot[k] is the slot of the k-th photon
+// that entered the flight box.  A taker forms the rest as the birth did: the
+// photon index from the slot, the position from the source and the box entry
+// t0 (0 when the source is not clipped), the energy from the table row, the
+// majorant from the label set.
 struct BirthBuffer {
-  float px[64], py[64], pz[64], dx[64], dy[64], dz[64], e[64], t1[64], mumax[64];
-  uint32_t id[64];
-  uint8_t slot[64];
+  float dx[64], dy[64], dz[64], t0[64], t1[64];
+  uint8_t eb[64], set[64], slot[64];
 };
 
 constexpr int kWaveStats = 10;
@@ -260,6 +263,8 @@ struct WaveTables {
   uint8_t guide[kGuide + 4];
   BirthBuffer bb;
 };
+// 6 waves per SIMD (24 per CU) fit in a CU's 160 KB of LDS
+static_assert(24 * sizeof(WaveTables) <= 160 * 1024, "WaveTables too large for 6 waves per SIMD");
 
 // Inverse CDF of the spectrum (CBCT_real325.cu:479-486): the first row e in
 // [1, kRows - 1] with cdf[e] >= u, or the last row, searched within [lo, hi].
@@ -336,15 +341,9 @@ constexpr uint32_t kAnyLabel = 16u;
 // leaves without entering the phantom box, or meets nothing that attenuates
 // (caller tallies it).  The Woodcock majorant is the largest mu among the
 // labels of `set` (bit l: label l in 0..3; kAnyLabel: every label).
-__device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables& wt, uint32_t set) {
-  // The majorant is formed before the box test so that `set` (a global load at a
-  // birth) is consumed on every path: a load left pending on the early-exit path
-  // made the compiler wait for all memory operations, tally atomics included,
-  // at the start of every Woodcock step.
-  p.mu0 = wt.mu[0][p.eb];
-  p.mu1 = wt.mu[1][p.eb];
-  p.mu2 = wt.mu[2][p.eb];
-  p.mu3 = wt.mu[3][p.eb];
+// The largest mu among the labels of `set` (bit l: label l in 0..3;
+// kAnyLabel: every label) at the photon's table row, from its mu registers.
+__device__ inline float set_majorant(const Photon& p, const WaveTables& wt, uint32_t set) {
   float mm;
   if (set & kAnyLabel) {
     mm = wt.mumax[p.eb];
@@ -355,12 +354,28 @@ __device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables&
     if (set & 4u) mm = fmaxf(mm, p.mu2);
     if (set & 8u) mm = fmaxf(mm, p.mu3);
   }
+  return mm;
+}
+
+// (t_in: the distance moved to the box entry, 0 if the photon starts inside.)
+__device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables& wt, uint32_t set, float& t_in) {
+  // The majorant is formed before the box test so that `set` (a global load at a
+  // birth) is consumed on every path: a load left pending on the early-exit path
+  // made the compiler wait for all memory operations, tally atomics included,
+  // at the start of every Woodcock step.
+  p.mu0 = wt.mu[0][p.eb];
+  p.mu1 = wt.mu[1][p.eb];
+  p.mu2 = wt.mu[2][p.eb];
+  p.mu3 = wt.mu[3][p.eb];
+  const float mm = set_majorant(p, wt, set);
   if (!(mm > 0.f)) return false;
   float t0, t1;
   if (!ray_box(p.px, p.py, p.pz, p.dx, p.dy, p.dz, J, t0, t1) || !(t1 > 0.f)) return false;
+  t_in = 0.f;
   if (t0 > 0.f) {
     p.px = fmaf(t0, p.dx, p.px); p.py = fmaf(t0, p.dy, p.py); p.pz = fmaf(t0, p.dz, p.pz);
     t1 -= t0;
+    t_in = t0;
   }
   p.t = 0.f;
   p.t1 = t1;
@@ -448,6 +463,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   uint32_t buf_head = 0, buf_n = 0;  // the birth buffer's next photon and photons left (wave-uniform)
   int buf_view = 0;                  // the view of the buffered photons, and its cos and sin
   float buf_cv = 0.f, buf_sv = 0.f;
+  uint32_t buf_id0 = 0;              // photon index of the buffer's slot 0 (the tile's corner pixel)
 
   int state = kDead;
   Photon p{};
@@ -593,12 +609,12 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           q.e = e;
           q.eb = ebin_of(e);
           q.nint = 0;
-          born = start_flight(q, *Jp, wt, set);
+          float tin;
+          born = start_flight(q, *Jp, wt, set, tin);
           if (born) {
-            wt.bb.px[lane] = q.px; wt.bb.py[lane] = q.py; wt.bb.pz[lane] = q.pz;
             wt.bb.dx[lane] = q.dx; wt.bb.dy[lane] = q.dy; wt.bb.dz[lane] = q.dz;
-            wt.bb.e[lane] = q.e; wt.bb.t1[lane] = q.t1; wt.bb.mumax[lane] = q.mumax;
-            wt.bb.id[lane] = id;
+            wt.bb.t0[lane] = tin; wt.bb.t1[lane] = q.t1;
+            wt.bb.eb[lane] = (uint8_t)q.eb; wt.bb.set[lane] = (uint8_t)set;
           } else {
             bcode = tally(q, *Jp);
           }
@@ -616,6 +632,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         buf_view = unit_view;
         buf_cv = cv;
         buf_sv = sv;
+        buf_id0 = (uint32_t)((uint64_t)unit_cyc * npix + tv * kTileV * (uint32_t)Jp->nu + tu * kTileU);
         if (++unit_g == unit_nt) {  // the unit's tiles are done for this cycle
           unit_g = 0;
           ++unit_cyc;
@@ -625,25 +642,29 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
       if (state == kDead && rank < buf_n) {
         const int s = wt.bb.slot[buf_head + rank];
-        p.px = wt.bb.px[s]; p.py = wt.bb.py[s]; p.pz = wt.bb.pz[s];
+        // as the birth formed them (start_flight's box clip, majorant and scale)
         p.dx = wt.bb.dx[s]; p.dy = wt.bb.dy[s]; p.dz = wt.bb.dz[s];
-        p.e = wt.bb.e[s];
-        p.eb = ebin_of(p.e);
+        const float tin = wt.bb.t0[s];
+        p.px = fmaf(tin, p.dx, -Jp->sod * buf_cv);
+        p.py = fmaf(tin, p.dy, -Jp->sod * buf_sv);
+        p.pz = fmaf(tin, p.dz, 0.f);
+        p.eb = wt.bb.eb[s];
+        p.e = Jp->spectrum_cdf ? (float)p.eb : Jp->e_kev;  // a spectrum row is its energy in keV
         p.t = 0.f;
         p.t1 = wt.bb.t1[s];
-        p.mumax = wt.bb.mumax[s];
-        p.lscale = 0.69314718055994531f * rcp(p.mumax);  // as start_flight forms it
         p.mu0 = wt.mu[0][p.eb];
         p.mu1 = wt.mu[1][p.eb];
         p.mu2 = wt.mu[2][p.eb];
         p.mu3 = wt.mu[3][p.eb];
+        p.mumax = set_majorant(p, wt, wt.bb.set[s]);
+        p.lscale = 0.69314718055994531f * rcp(p.mumax);
         p.nint = 0;
         p.view = buf_view;
         p.cv = buf_cv;
         p.sv = buf_sv;
         // the spectrum draw took block 0 of the photon's stream; flights take whole blocks
         rs.ctr = Jp->spectrum_cdf ? 1u : 0u;
-        rs.photon = wt.bb.id[s];
+        rs.photon = buf_id0 + (uint32_t)(s / kTileU) * (uint32_t)Jp->nu + (uint32_t)(s % kTileU);
         rs.view = (uint32_t)buf_view;
         rs.left = 0;
         state = kFly;
@@ -689,7 +710,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             rotate_dir(p.dx, p.dy, p.dz, cth, w);
           }
           p.nint += 1;
-          if (start_flight(p, *Jp, wt, kAnyLabel)) {
+          float tin_;
+          if (start_flight(p, *Jp, wt, kAnyLabel, tin_)) {
             state = kFly;
           } else {
             icode = tally(p, *Jp);
