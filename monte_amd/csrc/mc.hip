@@ -240,16 +240,18 @@ struct Photon {
 constexpr int kGuide = 256;  // spectrum guide cells
 
 // Photons born 64 at a time, one per lane with every lane active, wait here
-// until dead lanes take them (ray-box clip and label set already done; all are
-// of one view and one cycle of one detector tile).  Slots are the lanes that
-// made them, i.e. the tile's pixels; slot[k] is the slot of the k-th photon
-// that entered the flight box.  A taker forms the rest as the birth did: the
-// photon index from the slot, the position from the source and the box entry
-// t0 (0 when the source is not clipped), the energy from the table row, the
-// majorant from the label set.
+// until dead lanes take them; all are of one view and one cycle of one
+// detector tile, and slots are the lanes that made them, i.e. the tile's
+// pixels.  The rays of the tile's pixels in that view (direction, flight-box
+// entry t0 (0 when the source is not clipped) and exit t1 (-1: no flight),
+// label set) are formed once per tile and view and serve every cycle of it;
+// per photon there is only the table row eb.  slot[k] is the slot of the k-th
+// photon that entered the flight box.  A taker forms the rest as the birth
+// did: the photon index from the slot, the position from the source and t0,
+// the energy from the table row, the majorant from the label set.
 struct BirthBuffer {
   float dx[64], dy[64], dz[64], t0[64], t1[64];
-  uint8_t eb[64], set[64], slot[64];
+  uint8_t set[64], eb[64], slot[64];
 };
 
 constexpr int kWaveStats = 10;
@@ -464,6 +466,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   int buf_view = 0;                  // the view of the buffered photons, and its cos and sin
   float buf_cv = 0.f, buf_sv = 0.f;
   uint32_t buf_id0 = 0;              // photon index of the buffer's slot 0 (the tile's corner pixel)
+  uint32_t ray_tile = ~0u;           // the tile and view whose rays the buffer holds (~0u: none)
+  int ray_view = -1;
 
   int state = kDead;
   Photon p{};
@@ -530,6 +534,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           if (switched) {
             if (job >= 0) flush(counters + job);
             job = pj;
+            ray_tile = ~0u;  // another job's geometry
           }
           // units run views fastest, then cycle blocks, then tile groups: the units in
           // flight share a few detector tiles, hence a thin z-slab of the phantom that
@@ -576,27 +581,44 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         const uint32_t id = (uint32_t)n64;
         CU32* vcs = (CU32*)Jp->view_cs + 2 * unit_view;  // read-only during the launch: scalar loads
         const float cv = __uint_as_float(vcs[0]), sv = __uint_as_float(vcs[1]);
-        bool born = false;
-        int bcode = -1;  // tally code of a photon that misses the flight box
-        const bool valid = u_ < (uint32_t)Jp->nu && v_ < (uint32_t)Jp->nv && n64 < Jp->photons_per_view;
-        if (valid) {
-          // the first flight's majorant comes from the labels its line crosses;
-          // loaded first, so the load is in flight during the spectrum draw
-          const uint32_t set = Jp->ray_set
-                                   ? (uint32_t)Jp->ray_set[(size_t)(unit_view - Jp->view_begin) * npix + pix]
-                                   : kAnyLabel;
+        const bool on_det = u_ < (uint32_t)Jp->nu && v_ < (uint32_t)Jp->nv;
+        if (t != ray_tile || unit_view != ray_view) {
+          // the tile's rays in this view, one per lane: the first flight runs from the
+          // source towards the pixel centre, its majorant comes from the labels the
+          // line crosses, and it starts at the flight box (start_flight's clip)
+          const uint32_t set =
+              on_det && Jp->ray_set ? (uint32_t)Jp->ray_set[(size_t)(unit_view - Jp->view_begin) * npix + pix]
+                                    : kAnyLabel;
           const float yp = ((float)u_ + 0.5f - Jp->un0) * Jp->u_step;
           const float zp = ((float)v_ + 0.5f - Jp->vn0) * Jp->v_step;
-          const float in = rsqrtf(Jp->sdd * Jp->sdd + yp * yp + zp * zp);
+          const float in = rsqrtf(fmaf(zp, zp, fmaf(Jp->sdd, Jp->sdd, yp * yp)));
+          // the fused products are spelled out, so the direction's rounding does not
+          // depend on which product the compiler would fuse here
+          const float dx = fmaf(Jp->sdd, cv, -(yp * sv)) * in;
+          const float dy = fmaf(cv, yp, Jp->sdd * sv) * in;
+          const float dz = zp * in;
+          float t0, t1;
+          const bool hit = ray_box(-Jp->sod * cv, -Jp->sod * sv, 0.f, dx, dy, dz, *Jp, t0, t1) && t1 > 0.f;
+          float tin = 0.f;
+          if (t0 > 0.f) {
+            t1 -= t0;
+            tin = t0;
+          }
+          wt.bb.dx[lane] = dx; wt.bb.dy[lane] = dy; wt.bb.dz[lane] = dz;
+          wt.bb.t0[lane] = tin; wt.bb.t1[lane] = hit ? t1 : -1.f;
+          wt.bb.set[lane] = (uint8_t)set;
+          ray_tile = t;
+          ray_view = unit_view;
+        }
+        bool born = false;
+        int bcode = -1;  // tally code of a photon that misses the flight box
+        const bool valid = on_det && n64 < Jp->photons_per_view;
+        if (valid) {
           Photon q;
           q.view = unit_view;
           q.cv = cv;
           q.sv = sv;
-          // the fused products are spelled out, so the direction's rounding does not
-          // depend on which product the compiler would fuse here
-          q.dx = fmaf(Jp->sdd, cv, -(yp * sv)) * in;
-          q.dy = fmaf(cv, yp, Jp->sdd * sv) * in;
-          q.dz = zp * in;
+          q.dx = wt.bb.dx[lane]; q.dy = wt.bb.dy[lane]; q.dz = wt.bb.dz[lane];
           q.px = -Jp->sod * cv; q.py = -Jp->sod * sv; q.pz = 0.f;
           float e = Jp->e_kev;
           if (Jp->spectrum_cdf) {
@@ -609,12 +631,14 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           q.e = e;
           q.eb = ebin_of(e);
           q.nint = 0;
-          float tin;
-          born = start_flight(q, *Jp, wt, set, tin);
+          // start_flight's tests: something attenuates, and the line meets the box
+          q.mu0 = wt.mu[0][q.eb];
+          q.mu1 = wt.mu[1][q.eb];
+          q.mu2 = wt.mu[2][q.eb];
+          q.mu3 = wt.mu[3][q.eb];
+          born = set_majorant(q, wt, wt.bb.set[lane]) > 0.f && wt.bb.t1[lane] >= 0.f;
           if (born) {
-            wt.bb.dx[lane] = q.dx; wt.bb.dy[lane] = q.dy; wt.bb.dz[lane] = q.dz;
-            wt.bb.t0[lane] = tin; wt.bb.t1[lane] = q.t1;
-            wt.bb.eb[lane] = (uint8_t)q.eb; wt.bb.set[lane] = (uint8_t)set;
+            wt.bb.eb[lane] = (uint8_t)q.eb;
           } else {
             bcode = tally(q, *Jp);
           }
