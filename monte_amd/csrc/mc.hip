@@ -301,13 +301,13 @@ __device__ inline void count_lanes(WaveTables& wt, int k, bool pred) {
 enum { kTallyMiss = 0, kTallyPrimary = 1, kTallyScatter = 2 };
 
 // Counts the tallies of a section from each lane's tally code (-1: none).
+// Misses are not counted here: every history ends once, tallied, missed,
+// absorbed or capped, so the flush derives them.
 __device__ inline void count_tallies(WaveTables& wt, int code) {
-  const uint64_t mp = __ballot(code == kTallyPrimary), ms = __ballot(code == kTallyScatter),
-                 mm = __ballot(code == kTallyMiss);
-  if (__lane_id() == 0 && (mp | ms | mm) != 0) {
+  const uint64_t mp = __ballot(code == kTallyPrimary), ms = __ballot(code == kTallyScatter);
+  if (__lane_id() == 0 && (mp | ms) != 0) {
     atomicAdd(&wt.stats[kStatTp], (unsigned long long)popc(mp));
     atomicAdd(&wt.stats[kStatTs], (unsigned long long)popc(ms));
-    atomicAdd(&wt.stats[kStatMissed], (unsigned long long)popc(mm));
   }
 }
 
@@ -479,7 +479,9 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   extern __shared__ WaveTables wave_tables[];  // one slot per wave of the block
   WaveTables& wt = wave_tables[threadIdx.x >> 6];
   if (lane < kStats) wt.stats[lane] = 0;
-  // adds the wave's counters to the job's and clears them
+  // adds the wave's counters to the job's and clears them; called once all the
+  // histories the wave began for the job have ended, so the misses and the
+  // interactions follow from the other counts
   auto flush = [&](McCounters* dst) {
     if (__lane_id() == 0) {
       unsigned long long v[kStats];
@@ -488,6 +490,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
         v[k] = wt.stats[k];
         wt.stats[k] = 0;
       }
+      v[kStatMissed] = v[kStatHistories] - v[kStatTp] - v[kStatTs] - v[kStatPhoto] - v[kStatCapped];
+      v[kStatInteractions] = v[kStatPhoto] + v[kStatCoherent] + v[kStatCompton];
       atomicAdd(&dst->histories, v[kStatHistories]); atomicAdd(&dst->steps, v[kStatSteps]);
       atomicAdd(&dst->interactions, v[kStatInteractions]); atomicAdd(&dst->photo, v[kStatPhoto]);
       atomicAdd(&dst->coherent, v[kStatCoherent]); atomicAdd(&dst->compton, v[kStatCompton]);
@@ -743,7 +747,6 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           }
         }
       }
-      count_lanes(wt, kStatInteractions, ev != 0);
       count_lanes(wt, kStatPhoto, ev == 1);
       count_lanes(wt, kStatCoherent, ev == 2);
       count_lanes(wt, kStatCompton, ev == 3);
