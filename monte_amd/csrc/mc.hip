@@ -278,7 +278,9 @@ __device__ inline int spectrum_row(const float* cdf, float u, int lo, int hi) {
   return lo;
 }
 
-enum { kDead = 0, kFly = 1, kCollide = 2 };
+// kTally: the flight ended; the photon waits to be tallied with the next refill
+// batch (its position and direction do not change meanwhile)
+enum { kDead = 0, kFly = 1, kCollide = 2, kTally = 3 };
 // Statistics are counted per wave, not per lane: where every lane of the wave
 // is active, a ballot of the lanes where an event happened is added to the
 // wave's 64-bit counter in LDS, and the counters go to the job's when it is
@@ -503,9 +505,10 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   for (;;) {
     const McJob J = load_job(cjobs, __builtin_amdgcn_readfirstlane(job < 0 ? 0 : job));
     const McJob* Jp = &J;
-    uint64_t dead = __ballot(state == kDead);
+    uint64_t dead = __ballot(state == kDead || state == kTally);
     const uint64_t fly = __ballot(state == kFly);
     const uint64_t coll = __ballot(state == kCollide);
+    const uint64_t pend = __ballot(state == kTally);
     const bool all_dead = (fly | coll) == 0;
 
     // 1) secure a work unit when lanes are idle and the current one is spent
@@ -533,7 +536,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       }
       if (pending >= 0) {
         const int pj = __builtin_amdgcn_readfirstlane(job_of_unit(cjobs, n_jobs, (uint64_t)pending));
-        if (pj == job || (all_dead && buf_n == 0)) {  // buffered photons are of the current job
+        if (pj == job || (all_dead && buf_n == 0 && pend == 0)) {  // buffered and ended photons are of the current job
           const bool switched = pj != job;
           if (switched) {
             if (job >= 0) flush(counters + job);
@@ -566,6 +569,16 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           }
         }
       }
+    }
+    // 1b) tallies of the flights that ended, batched with the refills (all run when
+    //     nothing flies, so none is left when the wave stops or changes job)
+    if (pend != 0 && (popc(dead) >= kRefillBatch || fly == 0)) {
+      int code = -1;
+      if (state == kTally) {
+        code = tally(p, *Jp);
+        state = kDead;
+      }
+      count_tallies(wt, code);
     }
     if (all_dead && unit_cyc >= unit_cyc_end && pending < 0 && exhausted && buf_n == 0) break;
 
@@ -707,7 +720,6 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       count_lanes(wt, kStatCapped, cap);
       if (cap) state = kDead;
       int ev = 0;      // 1 photoelectric, 2 coherent, 3 Compton
-      int icode = -1;  // tally code of a scattered photon that leaves the flight box
       if (state == kCollide) {
         const float4 m = Jp->tables->m[p.lab][p.eb];
         rs.left = 0;  // interactions start on a fresh block
@@ -742,15 +754,13 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           if (start_flight(p, *Jp, wt, kAnyLabel, tin_)) {
             state = kFly;
           } else {
-            icode = tally(p, *Jp);
-            state = kDead;
+            state = kTally;
           }
         }
       }
       count_lanes(wt, kStatPhoto, ev == 1);
       count_lanes(wt, kStatCoherent, ev == 2);
       count_lanes(wt, kStatCompton, ev == 3);
-      count_tallies(wt, icode);
     }
 
     // 4) two Woodcock steps for every flying lane from one Philox block
@@ -758,7 +768,6 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
     //    before either is used, and a flight ending on the first step leaves
     //    the block's other two uniforms unused.
     count_lanes(wt, kStatSteps, state == kFly);
-    int scode = -1;     // tally code of a flight that ends
     bool two = false;   // took the second step of the pair
     if (state == kFly) {
       uint32_t c0 = rs.ctr, c1 = rs.photon, c2 = rs.view, c3 = Jp->job_id;
@@ -788,8 +797,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       asm volatile("" ::"v"(mua), "v"(mub));
       const bool real_a = u01(c1) * p.mumax < mua, real_b = u01(c3) * p.mumax < mub;
       if (ta >= p.t1) {
-        scode = tally(p, *Jp);
-        state = kDead;
+        state = kTally;
       } else if (real_a) {  // real collision: park until the wave runs interactions
         p.px = ax; p.py = ay; p.pz = az;
         p.lab = la;
@@ -797,8 +805,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       } else {
         two = true;
         if (tb >= p.t1) {
-          scode = tally(p, *Jp);
-          state = kDead;
+          state = kTally;
         } else if (real_b) {
           p.px = bx; p.py = by; p.pz = bz;
           p.lab = lb;
@@ -809,7 +816,6 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       }
     }
     count_lanes(wt, kStatSteps, two);
-    count_tallies(wt, scode);
   }
   if (job >= 0) flush(counters + job);
 }
