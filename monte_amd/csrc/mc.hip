@@ -186,7 +186,8 @@ __device__ inline bool slab(float p, float d, float b0, float b1, float& lo, flo
 }
 
 // Parametric interval of p + t d inside the phantom box.
-__device__ inline bool ray_box(float px, float py, float pz, float dx, float dy, float dz, const McJob& J,
+template <class JT>
+__device__ inline bool ray_box(float px, float py, float pz, float dx, float dy, float dz, const JT& J,
                                float& t0, float& t1) {
   float lo = -INFINITY, hi = INFINITY;
   const bool ok = slab(px, dx, J.bx0, J.bx1, lo, hi) && slab(py, dy, J.by0, J.by1, lo, hi) &&
@@ -198,17 +199,6 @@ __device__ inline bool ray_box(float px, float py, float pz, float dx, float dy,
 
 using CJob = const __attribute__((address_space(4))) McJob;
 using CU32 = const __attribute__((address_space(4))) uint32_t;
-
-// Job i read through the constant address space (scalar loads).
-__device__ inline McJob load_job(const CJob* jobs, int i) {
-  static_assert(sizeof(McJob) % 4 == 0, "McJob is copied as dwords");
-  McJob j;
-  CU32* src = (CU32*)(jobs + i);
-  uint32_t* dst = (uint32_t*)&j;
-#pragma unroll
-  for (int k = 0; k < (int)(sizeof(McJob) / 4); ++k) dst[k] = src[k];
-  return j;
-}
 
 __device__ inline int job_of_unit(const CJob* jobs, int n_jobs, uint64_t u) {
   int lo = 0, hi = n_jobs - 1;
@@ -313,7 +303,8 @@ __device__ inline void count_tallies(WaveTables& wt, int code) {
   }
 }
 
-__device__ inline int tally(const Photon& p, const McJob& J) {
+template <class JT>
+__device__ inline int tally(const Photon& p, const JT& J) {
   const float cx = p.cv, cy = p.sv;
   const float xq = p.px * cx + p.py * cy, yq = -p.px * cy + p.py * cx;
   const float ddx = p.dx * cx + p.dy * cy, ddy = -p.dx * cy + p.dy * cx;
@@ -362,7 +353,8 @@ __device__ inline float set_majorant(const Photon& p, const WaveTables& wt, uint
 }
 
 // (t_in: the distance moved to the box entry, 0 if the photon starts inside.)
-__device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables& wt, uint32_t set, float& t_in) {
+template <class JT>
+__device__ inline bool start_flight(Photon& p, const JT& J, const WaveTables& wt, uint32_t set, float& t_in) {
   // The majorant is formed before the box test so that `set` (a global load at a
   // birth) is consumed on every path: a load left pending on the early-exit path
   // made the compiler wait for all memory operations, tally atomics included,
@@ -392,7 +384,8 @@ __device__ inline bool start_flight(Photon& p, const McJob& J, const WaveTables&
 // The host builds the CDF from non-negative weights, so it is non-decreasing
 // and the search within a guide cell returns the row a search over all rows
 // would.
-__device__ inline void load_wave_tables(WaveTables& wt, const McJob& J, int lane) {
+template <class JT>
+__device__ inline void load_wave_tables(WaveTables& wt, const JT& J, int lane) {
   for (int e = lane; e < kRows; e += 64) {
     wt.cdf[e] = J.spectrum_cdf ? J.spectrum_cdf[e] : 0.f;
     wt.mumax[e] = J.tables->mumax[e];
@@ -408,7 +401,8 @@ __device__ inline void load_wave_tables(WaveTables& wt, const McJob& J, int lane
 }
 
 // Attenuation at a label: labels 0..3 from registers, others from the table.
-__device__ inline float mu_at(const Photon& p, int lab, const McJob& J) {
+template <class JT>
+__device__ inline float mu_at(const Photon& p, int lab, const JT& J) {
   const bool odd = (lab & 1) != 0;
   const float lo = odd ? p.mu1 : p.mu0, hi = odd ? p.mu3 : p.mu2;
   float mu = (lab & 2) ? hi : lo;
@@ -425,7 +419,8 @@ __device__ inline uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 // Position at flight parameter t and the index of its voxel.
-__device__ inline uint32_t voxel_at(const Photon& p, float t, const McJob& J, float& qx, float& qy, float& qz) {
+template <class JT>
+__device__ inline uint32_t voxel_at(const Photon& p, float t, const JT& J, float& qx, float& qy, float& qz) {
   qx = fmaf(t, p.dx, p.px); qy = fmaf(t, p.dy, p.py); qz = fmaf(t, p.dz, p.pz);
   // voxel index = (q - b0) / v as one FMA; truncation equals floor wherever
   // the clamp to [0, n - 1] does not already decide it
@@ -439,7 +434,8 @@ __device__ inline uint32_t voxel_at(const Photon& p, float t, const McJob& J, fl
 
 // Label byte of a voxel, by a global-address-space load; the caller clamps
 // labels past the table (min(label, kMaxLabels - 1)) so no table read leaves it.
-__device__ inline uint32_t label_raw(uint32_t i, const McJob& J) {
+template <class JT>
+__device__ inline uint32_t label_raw(uint32_t i, const JT& J) {
   return ((const __attribute__((address_space(1))) uint8_t*)J.labels)[i];
 }
 
@@ -503,8 +499,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   };
 
   for (;;) {
-    const McJob J = load_job(cjobs, __builtin_amdgcn_readfirstlane(job < 0 ? 0 : job));
-    const McJob* Jp = &J;
+    // fields are read where they are used, by scalar loads
+    CJob* Jp = cjobs + __builtin_amdgcn_readfirstlane(job < 0 ? 0 : job);
     uint64_t dead = __ballot(state == kDead || state == kTally);
     const uint64_t fly = __ballot(state == kFly);
     const uint64_t coll = __ballot(state == kCollide);
@@ -563,7 +559,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           }
           pending = -1;
           if (switched) {  // J is re-read for the new job at the top of the loop
-            load_wave_tables(wt, load_job(cjobs, pj), lane);
+            load_wave_tables(wt, cjobs[pj], lane);
             __builtin_amdgcn_wave_barrier();
             continue;
           }
