@@ -418,15 +418,23 @@ __device__ inline uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
   return r;
 }
 
+// min(max(x, 0), hi) for hi >= 0 as one median-of-three (the compiler forms
+// it only when hi is a constant)
+__device__ inline int clamp_index(int x, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(x), "s"(hi));
+  return r;
+}
+
 // Position at flight parameter t and the index of its voxel.
 template <class JT>
 __device__ inline uint32_t voxel_at(const Photon& p, float t, const JT& J, float& qx, float& qy, float& qz) {
   qx = fmaf(t, p.dx, p.px); qy = fmaf(t, p.dy, p.py); qz = fmaf(t, p.dz, p.pz);
   // voxel index = (q - b0) / v as one FMA; truncation equals floor wherever
   // the clamp to [0, n - 1] does not already decide it
-  const int ix = min(max((int)fmaf(qx, J.inv_vx, J.ix0), 0), J.nx - 1);
-  const int iy = min(max((int)fmaf(qy, J.inv_vy, J.iy0), 0), J.ny - 1);
-  const int iz = min(max((int)fmaf(qz, J.inv_vz, J.iz0), 0), J.nz - 1);
+  const int ix = clamp_index((int)fmaf(qx, J.inv_vx, J.ix0), J.nx - 1);
+  const int iy = clamp_index((int)fmaf(qy, J.inv_vy, J.iy0), J.ny - 1);
+  const int iz = clamp_index((int)fmaf(qz, J.inv_vz, J.iz0), J.nz - 1);
   // 32-bit index from two 24-bit multiply-adds: the host checks
   // nz * ny < 2^24, nx < 2^24 and nx * ny * nz < 2^32
   return mad_u24(mad_u24((uint32_t)iz, (uint32_t)J.ny, (uint32_t)iy), (uint32_t)J.nx, (uint32_t)ix);
