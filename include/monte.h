@@ -139,7 +139,8 @@ typedef struct monte_mc_job {
 /* Launch parameters picked by monte_autotune (or set by hand). */
 typedef struct monte_tuning {
   int32_t mc_block;          /* threads per block of the transport kernel */
-  int32_t mc_batch;          /* histories per lane per work-queue batch */
+  int32_t mc_batch;          /* photon cycles per work unit of one 8x8 detector tile (about 64 * mc_batch
+                                histories per unit; views with fewer cycles group tiles) */
   int32_t mc_blocks_per_cu;  /* persistent grid = CUs * this */
   int32_t bp_zchunk;         /* voxels along z per backprojection thread */
   int32_t bp_tile_x, bp_tile_y; /* backprojection block tile (threads) */
