@@ -289,24 +289,39 @@ __device__ inline bool wrap_source(int k, int r, int c, int n_views, int nu, int
   return kk < n_views;
 }
 
+// One thread per guard cell (the interior is left alone): per view, the guard
+// rows above the detector, the guard columns of its nv rows, the guard rows below.
 __global__ void fill_pads_kernel(float* __restrict__ buf, int n_views, int nu, int nv, int pitch, int rows,
                                  int wrap) {
-  const size_t n = (size_t)n_views * rows * pitch;
+  const int top = kPadLo * pitch, side = pitch - nu, mid = nv * side;
+  const int per_view = rows * pitch - nu * nv;
+  const size_t n = (size_t)n_views * per_view;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int k = (int)(i / ((size_t)rows * pitch));
-    const int rem = (int)(i - (size_t)k * rows * pitch);
-    const int r = rem / pitch - kPadLo, c = rem % pitch - kPadLo;
-    if (r >= 0 && r < nv && c >= 0 && c < nu) continue;
+    const int k = (int)(i / per_view);
+    int j = (int)(i - (size_t)k * per_view);
+    int r, c;  // detector coordinates of the guard cell
+    if (j < top) {
+      r = j / pitch - kPadLo;
+      c = j % pitch - kPadLo;
+    } else if ((j -= top) < mid) {
+      r = j / side;
+      const int q = j - r * side;
+      c = q < kPadLo ? q - kPadLo : nu + (q - kPadLo);
+    } else {
+      j -= mid;
+      r = nv + j / pitch;
+      c = j % pitch - kPadLo;
+    }
     float v = 0.f;
     int kk, rr, cc;
     if (wrap && wrap_source(k, r, c, n_views, nu, nv, kk, rr, cc))
       v = buf[((size_t)kk * rows + rr + kPadLo) * pitch + cc + kPadLo];
-    buf[i] = v;
+    buf[((size_t)k * rows + r + kPadLo) * pitch + c + kPadLo] = v;
   }
 }
 
 hipError_t launch_fill_pads(float* buf, int n_views, int nu, int nv, int pitch, int rows, int wrap, hipStream_t s) {
-  const size_t n = (size_t)n_views * rows * pitch;
+  const size_t n = (size_t)n_views * ((size_t)rows * pitch - (size_t)nu * nv);
   size_t blocks = (n + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(fill_pads_kernel, dim3((unsigned)blocks), dim3(256), 0, s, buf, n_views, nu, nv, pitch, rows,
