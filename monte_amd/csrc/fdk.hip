@@ -548,27 +548,29 @@ static void bp_tiling(const BpGrid& grid, const monte_tuning& t, int zc, int& ti
 // p[r + 1][c + 1] - p[r + 1][c]) of the padded filtered view p (zero past the last
 // row or column): the two rows of a voxel's bilinear taps with their u
 // differences, which the backprojector would otherwise form per voxel.
-__global__ void tap_quads_kernel(const float* __restrict__ pad, float4* __restrict__ quads, int rows, int pitch,
-                                 size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const size_t per_view = (size_t)rows * pitch;
-    const size_t k = i / per_view;
-    const int rc = (int)(i - k * per_view);
-    const int r = rc / pitch, cl = rc - r * pitch;
-    const float* p = pad + i;
-    const bool rr = r + 1 < rows, cr = cl + 1 < pitch;
-    const float p00 = p[0], p01 = cr ? p[1] : 0.f;
-    const float p10 = rr ? p[pitch] : 0.f, p11 = (rr && cr) ? p[pitch + 1] : 0.f;
-    quads[i] = make_float4(p00, p01 - p00, p10, p11 - p10);
-  }
+// One thread per padded cell: blocks of 64 columns x 4 rows, one grid layer
+// per view, so the cell's view, row and column need no division.
+constexpr int kQuadCols = 64, kQuadRows = 4;
+__global__ __launch_bounds__(kQuadCols * kQuadRows) void tap_quads_kernel(const float* __restrict__ pad,
+                                                                          float4* __restrict__ quads, int rows,
+                                                                          int pitch) {
+  const int cl = blockIdx.x * kQuadCols + (int)(threadIdx.x % kQuadCols);
+  const int r = blockIdx.y * kQuadRows + (int)(threadIdx.x / kQuadCols);
+  if (cl >= pitch || r >= rows) return;
+  const size_t i = ((size_t)blockIdx.z * rows + r) * pitch + cl;
+  const float* p = pad + i;
+  const bool rr = r + 1 < rows, cr = cl + 1 < pitch;
+  const float p00 = p[0], p01 = cr ? p[1] : 0.f;
+  const float p10 = rr ? p[pitch] : 0.f, p11 = (rr && cr) ? p[pitch + 1] : 0.f;
+  quads[i] = make_float4(p00, p01 - p00, p10, p11 - p10);
 }
 
 hipError_t launch_tap_quads(const float* filt_pad, float4* quads, int n_views, int rows, int pitch, hipStream_t s) {
-  const size_t n = (size_t)n_views * rows * pitch;
-  if (!n) return hipSuccess;
-  size_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(tap_quads_kernel, dim3((unsigned)blocks), dim3(256), 0, s, filt_pad, quads, rows, pitch, n);
+  if (n_views <= 0 || rows <= 0 || pitch <= 0) return hipSuccess;
+  if (n_views > 65535) return hipErrorInvalidValue;  // grid z limit
+  const dim3 grid((unsigned)((pitch + kQuadCols - 1) / kQuadCols), (unsigned)((rows + kQuadRows - 1) / kQuadRows),
+                  (unsigned)n_views);
+  hipLaunchKernelGGL(tap_quads_kernel, grid, dim3(kQuadCols * kQuadRows), 0, s, filt_pad, quads, rows, pitch);
   return hipGetLastError();
 }
 
