@@ -8,16 +8,18 @@
 //   * one photon per lane, its random numbers drawn from a Philox4x32-10
 //     stream keyed by (seed) with counter (block, photon, view, job), so any
 //     history can be replayed and results do not depend on scheduling;
-//   * the kernel is persistent: waves pull batches of photons (work units)
-//     from device work queues shared by all jobs of a launch (32 counters,
-//     each block drawing from its own first);
+//   * the kernel is persistent: waves pull work units (a detector tile of 8 x 8
+//     pixels over a block of photon cycles, in one view) from device work
+//     queues shared by all jobs of a launch (32 counters, each block drawing
+//     from its own first), so the units in flight share a thin slab of the
+//     phantom and few tally lines;
 //   * each loop iteration advances every flying lane by up to two Woodcock
 //     steps (one Philox block).
-//     Lanes that collide (real interaction pending) or die (absorbed, capped,
-//     tallied) park; the wave runs the interaction code, or refills dead lanes
-//     (ballot + mbcnt prefix ranks), only once enough lanes are parked or
-//     nothing else can run, so those branches execute near full width instead
-//     of for one lane at a time;
+//     Lanes that collide (real interaction pending), end their flight (tally
+//     pending) or die (absorbed, capped) park; the wave runs the interaction
+//     code, or tallies and refills (ballot + mbcnt prefix ranks), only once
+//     enough lanes are parked or nothing else can run, so those branches
+//     execute near full width instead of for one lane at a time;
 //   * photons are born 64 at a time, one per lane with every lane active, into
 //     a per-wave LDS buffer that dead lanes then take from;
 //   * flights are clipped to the bounding box of the voxels that attenuate
@@ -26,8 +28,9 @@
 //     starts at the box entry;
 //   * primary and scatter counts are tallied with FP64 atomics.  Every tally
 //     adds 1.0, so sums are exact integers and independent of order;
-//   * statistics are counted per lane where the events happen and summed over
-//     the wave when a job's counters are flushed.
+//   * statistics are counted per wave, from ballots of the lanes where events
+//     happen, into 64-bit counters in LDS that go to the job's when it is
+//     flushed.
 #include <algorithm>
 #include <climits>
 
@@ -332,10 +335,6 @@ __device__ inline int tally(const Photon& p, const JT& J) {
 // Label set meaning "any label": the flight takes the global majorant.
 constexpr uint32_t kAnyLabel = 16u;
 
-// Starts a flight from the current position; returns false if the photon
-// leaves without entering the phantom box, or meets nothing that attenuates
-// (caller tallies it).  The Woodcock majorant is the largest mu among the
-// labels of `set` (bit l: label l in 0..3; kAnyLabel: every label).
 // The largest mu among the labels of `set` (bit l: label l in 0..3;
 // kAnyLabel: every label) at the photon's table row, from its mu registers.
 __device__ inline float set_majorant(const Photon& p, const WaveTables& wt, uint32_t set) {
@@ -352,13 +351,14 @@ __device__ inline float set_majorant(const Photon& p, const WaveTables& wt, uint
   return mm;
 }
 
-// (t_in: the distance moved to the box entry, 0 if the photon starts inside.)
+// Starts a flight from the current position (after a scatter; a birth forms
+// its tile's rays the same way, once per tile and view); returns false if the
+// photon leaves without entering the flight box, or meets nothing that
+// attenuates (the caller tallies it).  The Woodcock majorant is the largest mu
+// among the labels of `set`.  t_in: the distance moved to the box entry, 0 if
+// the photon starts inside.
 template <class JT>
 __device__ inline bool start_flight(Photon& p, const JT& J, const WaveTables& wt, uint32_t set, float& t_in) {
-  // The majorant is formed before the box test so that `set` (a global load at a
-  // birth) is consumed on every path: a load left pending on the early-exit path
-  // made the compiler wait for all memory operations, tally atomics included,
-  // at the start of every Woodcock step.
   p.mu0 = wt.mu[0][p.eb];
   p.mu1 = wt.mu[1][p.eb];
   p.mu2 = wt.mu[2][p.eb];
