@@ -1208,15 +1208,18 @@ int monte_autotune(monte_ctx* c, int what, const monte_geom* g, const monte_phan
   if (what & 1) {
     if (!g || !ph || !cfg) return fail(c, MONTE_ERR_INVALID, "MC tuning needs geometry, phantom and config");
     // a run over all the job's views (units run views fastest, so the view
-    // count shapes the memory access pattern), about 2^28 photons in all and at
-    // least 2^16 per view, into scratch tallies: a launch of the full workload
-    // runs long enough that its last units' tail hardly counts, and the runs
-    // timed here must too (at 2^23 photons, launches of one wave per SIMD won
-    // and were 2.5x slower on the full workload)
+    // count shapes the memory access pattern) with 64 photon cycles per view
+    // where the job has them (so the units are tiles over 64 cycles, as in a
+    // long launch; 3 cycles of 2^28 photons over 360 views made units of 21
+    // tiles, and a slower choice) and at least 2^28 photons in all, into
+    // scratch tallies: the runs timed here must be long enough that the last
+    // units' tail hardly counts (at 2^23 photons, launches of one wave per SIMD
+    // won and were 2.5x slower on the full workload)
     monte_mc_cfg tc = *cfg;
     int vb = cfg->view_begin, ve = cfg->view_end;
     if (vb == 0 && ve == 0) ve = g->n_views;
-    const uint64_t per_view = std::max<uint64_t>(1u << 16, (1ull << 28) / (uint64_t)std::max(ve - vb, 1));
+    const uint64_t nvw = (uint64_t)std::max(ve - vb, 1), npix = (uint64_t)g->nu * (uint64_t)g->nv;
+    const uint64_t per_view = std::max<uint64_t>(64 * npix, (1ull << 28) / nvw);
     tc.photons_per_view = std::min<uint64_t>(cfg->photons_per_view, per_view);
     tc.view_begin = vb;
     tc.view_end = ve;
