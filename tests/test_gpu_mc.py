@@ -83,6 +83,28 @@ def test_mc_matches_oracle_small(ctx, oracle, coherent):
         assert abs(st[k] - st_ref[k]) <= max(10, 1e-3 * st_ref[k]), k
 
 
+@pytest.mark.parametrize("nu,nv,photons", [(13, 11, 100), (13, 11, 1000), (37, 21, 5000)])
+def test_mc_partial_tiles_and_cycles(ctx, oracle, nu, nv, photons):
+    """Work units are 8 x 8 detector tiles over photon cycles: detectors that end in partial
+    tiles, views with fewer photons than pixels (only the tile rows holding photons count) and
+    a partial last cycle must still run every photon once, as the oracle does."""
+    from monte_amd import geometry as G
+    from monte_amd import phantoms
+
+    geom = G.small(n_views=3).replace(nu=nu, nv=nv, du=0.8, dv=0.8)
+    ph = phantoms.small_phantom()
+    prim, scat, st, p_ref, s_ref, st_ref = _mc_pair(ctx, oracle, geom, ph, photons, e_kev=60.0)
+    print(nu, nv, photons, "gpu", st, "cpu", st_ref)
+    assert st["histories"] == st_ref["histories"] == photons * geom.n_views
+    # every history ends once: tallied, missed, absorbed or capped
+    assert st["tallied_primary"] + st["tallied_scatter"] + st["missed"] + st["photo"] + st["capped"] == \
+        st["histories"]
+    assert prim.sum() == st["tallied_primary"] and scat.sum() == st["tallied_scatter"]
+    # the same photons aim at the same pixels with the same draws
+    assert np.mean(prim == p_ref) > 0.99 and np.mean(scat == s_ref) > 0.95
+    assert abs(prim.sum() - p_ref.sum()) <= max(3, 1e-3 * p_ref.sum())
+
+
 def test_mc_primary_matches_beer_lambert(ctx, oracle):
     """Primary counts are Binomial(N0, exp(-L)) with L the exact line integral."""
     from monte_amd import geometry as G
