@@ -243,7 +243,8 @@ constexpr int kGuide = 256;  // spectrum guide cells
 // did: the photon index from the slot, the position from the source and t0,
 // the energy from the table row, the majorant from the label set.
 struct BirthBuffer {
-  float dx[64], dy[64], dz[64], t0[64], t1[64];
+  float4 d0[64];  // direction and box entry t0 of a slot's ray
+  float t1[64];
   uint8_t set[64], eb[64], slot[64];
 };
 
@@ -273,7 +274,7 @@ __device__ inline int spectrum_row(const float* cdf, float u, int lo, int hi) {
 
 // kTally: the flight ended; the photon waits to be tallied with the next refill
 // batch (its position and direction do not change meanwhile)
-enum { kDead = 0, kFly = 1, kCollide = 2, kTally = 3 };
+enum { kDead = 0, kTally = 1, kFly = 2, kCollide = 3 };  // dead or tally pending: state <= kTally
 // Statistics are counted per wave, not per lane: where every lane of the wave
 // is active, a ballot of the lanes where an event happened is added to the
 // wave's 64-bit counter in LDS, and the counters go to the job's when it is
@@ -448,10 +449,11 @@ __device__ inline uint32_t label_raw(uint32_t i, const JT& J) {
 }
 
 // A parked branch runs once at least this many lanes wait for it (or nothing
-// else can run).  Interactions wait for 24 lanes: each batch costs a whole
+// else can run).  Interactions wait for 32 lanes: each batch costs a whole
 // Klein-Nishina rejection loop and a flight set-up, so fuller batches pay
-// (20 to 28 measured equal, 1.3% faster than 16 on the full workload).
-constexpr unsigned kInteractBatch = 24;
+// (at 6 waves per SIMD, 16, 24 and 32 lanes took 96.7, 94.6 and 94.2 ms on the
+// full workload at 1e7 photons per view).
+constexpr unsigned kInteractBatch = 32;
 constexpr unsigned kRefillBatch = 16;
 
 __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restrict__ jobs, int n_jobs,
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
   for (;;) {
     // fields are read where they are used, by scalar loads
     CJob* Jp = cjobs + __builtin_amdgcn_readfirstlane(job < 0 ? 0 : job);
-    uint64_t dead = __ballot(state == kDead || state == kTally);
+    uint64_t dead = __ballot(state <= kTally);
     const uint64_t fly = __ballot(state == kFly);
     const uint64_t coll = __ballot(state == kCollide);
     const uint64_t pend = __ballot(state == kTally);
@@ -625,8 +627,8 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
             t1 -= t0;
             tin = t0;
           }
-          wt.bb.dx[lane] = dx; wt.bb.dy[lane] = dy; wt.bb.dz[lane] = dz;
-          wt.bb.t0[lane] = tin; wt.bb.t1[lane] = hit ? t1 : -1.f;
+          wt.bb.d0[lane] = make_float4(dx, dy, dz, tin);
+          wt.bb.t1[lane] = hit ? t1 : -1.f;
           wt.bb.set[lane] = (uint8_t)set;
           ray_tile = t;
           ray_view = unit_view;
@@ -639,7 +641,10 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
           q.view = unit_view;
           q.cv = cv;
           q.sv = sv;
-          q.dx = wt.bb.dx[lane]; q.dy = wt.bb.dy[lane]; q.dz = wt.bb.dz[lane];
+          {
+            const float4 d0 = wt.bb.d0[lane];
+            q.dx = d0.x; q.dy = d0.y; q.dz = d0.z;
+          }
           q.px = -Jp->sod * cv; q.py = -Jp->sod * sv; q.pz = 0.f;
           float e = Jp->e_kev;
           if (Jp->spectrum_cdf) {
@@ -688,8 +693,9 @@ __global__ __launch_bounds__(256) void mc_transport_kernel(const McJob* __restri
       if (state == kDead && rank < buf_n) {
         const int s = wt.bb.slot[buf_head + rank];
         // as the birth formed them (start_flight's box clip, majorant and scale)
-        p.dx = wt.bb.dx[s]; p.dy = wt.bb.dy[s]; p.dz = wt.bb.dz[s];
-        const float tin = wt.bb.t0[s];
+        const float4 d0 = wt.bb.d0[s];
+        p.dx = d0.x; p.dy = d0.y; p.dz = d0.z;
+        const float tin = d0.w;
         p.px = fmaf(tin, p.dx, -Jp->sod * buf_cv);
         p.py = fmaf(tin, p.dy, -Jp->sod * buf_sv);
         p.pz = fmaf(tin, p.dz, 0.f);
