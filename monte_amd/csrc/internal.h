@@ -305,10 +305,13 @@ hipError_t launch_fdk_filter_T(const float* filt_pad, float* proj, const float* 
 // The work queue is kMcQueues counters kMcQueueStride words apart, each
 // handing out one chunk (`chunk` consecutive work units) per round, and block
 // b draws from queue b mod kMcQueues first.  Blocks are dealt to the 8 XCDs
-// round robin, so an XCD draws from 4 of the queues and its L2 serves their
-// chunks; 32 counters rather than 8 halve the contention on each (8, 16 and
-// 32 queues took 164, 161 and 158 ms at 64 histories per lane per unit).
-constexpr int kMcQueues = 32;
+// round robin, so an XCD draws from 3 of the queues and its L2 serves their
+// chunks.  With units of detector tiles, 8, 16, 24, 32 and 64 queues took
+// 84.5, 84.2, 83.4, 88.0 and 105.8 ms at 360 views x 1e7 photons, and 24 took
+// 815.8 against 824.7 ms for 32 at 1e8 photons (fewer queues keep an XCD's
+// units on fewer tiles; with units of consecutive photons 8, 16 and 32 had
+// taken 164, 161 and 158 ms, the counters' contention then mattering more).
+constexpr int kMcQueues = 24;
 constexpr int kMcQueueStride = 16;
 constexpr size_t kMcQueueBytes = sizeof(unsigned int) * kMcQueues * kMcQueueStride;
 hipError_t launch_mc_transport(const McJob* jobs_dev, int n_jobs, uint64_t total_units, uint32_t chunk,

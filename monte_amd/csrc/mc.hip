@@ -10,7 +10,7 @@
 //     history can be replayed and results do not depend on scheduling;
 //   * the kernel is persistent: waves pull work units (a detector tile of 8 x 8
 //     pixels over a block of photon cycles, in one view) from device work
-//     queues shared by all jobs of a launch (32 counters, each block drawing
+//     queues shared by all jobs of a launch (24 counters, each block drawing
 //     from its own first), so the units in flight share a thin slab of the
 //     phantom and few tally lines;
 //   * each loop iteration advances every flying lane by up to two Woodcock
