@@ -207,7 +207,7 @@ int monte_mc_project(monte_ctx *ctx, const monte_geom *g, const monte_phantom *p
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Device
  * pointers only.  The launch's descriptors, attenuation tables, spectrum,
  * per-view table, work queue and counters live in a static 128 KiB device
- * array per GPU, which bounds the geometry to about 9300 views
+ * array per GPU, which bounds the geometry to about 9400 views
  * (MONTE_ERR_INVALID beyond); every call on a GPU reuses that GPU's array, so
  * calls that overlap on one GPU race.  Every flight takes the global Woodcock
  * majorant (the first-flight label sets of tuning mc_ray_brick need memory in
