@@ -83,18 +83,23 @@ def test_mc_matches_oracle_small(ctx, oracle, coherent):
         assert abs(st[k] - st_ref[k]) <= max(10, 1e-3 * st_ref[k]), k
 
 
-@pytest.mark.parametrize("nu,nv,photons", [(13, 11, 100), (13, 11, 1000), (37, 21, 5000)])
-def test_mc_partial_tiles_and_cycles(ctx, oracle, nu, nv, photons):
+@pytest.mark.parametrize("nu,nv,photons", [(13, 11, 100), (13, 11, 1000), (37, 21, 5000), (1, 1, 1), (1, 1, 50),
+                                            (9, 1, 7), (1, 9, 200)])
+@pytest.mark.parametrize("coherent", [0, 1])
+def test_mc_partial_tiles_and_cycles(ctx, oracle, nu, nv, photons, coherent):
     """Work units are 8 x 8 detector tiles over photon cycles: detectors that end in partial
     tiles, views with fewer photons than pixels (only the tile rows holding photons count) and
-    a partial last cycle must still run every photon once, as the oracle does."""
+    a partial last cycle must still run every photon once, as the oracle does.  The degenerate
+    cases are one pixel with one photon or many cycles, and one-row or one-column detectors.
+    Forward coherent scattering (coherent = 0) keeps the direction, so scattered photons reach
+    even a one-pixel detector and the scatter images are compared too."""
     from monte_amd import geometry as G
     from monte_amd import phantoms
 
     geom = G.small(n_views=3).replace(nu=nu, nv=nv, du=0.8, dv=0.8)
     ph = phantoms.small_phantom()
-    prim, scat, st, p_ref, s_ref, st_ref = _mc_pair(ctx, oracle, geom, ph, photons, e_kev=60.0)
-    print(nu, nv, photons, "gpu", st, "cpu", st_ref)
+    prim, scat, st, p_ref, s_ref, st_ref = _mc_pair(ctx, oracle, geom, ph, photons, e_kev=60.0, coherent=coherent)
+    print(nu, nv, photons, coherent, "gpu", st, "cpu", st_ref)
     assert st["histories"] == st_ref["histories"] == photons * geom.n_views
     # every history ends once: tallied, missed, absorbed or capped
     assert st["tallied_primary"] + st["tallied_scatter"] + st["missed"] + st["photo"] + st["capped"] == \
@@ -103,6 +108,29 @@ def test_mc_partial_tiles_and_cycles(ctx, oracle, nu, nv, photons):
     # the same photons aim at the same pixels with the same draws
     assert np.mean(prim == p_ref) > 0.99 and np.mean(scat == s_ref) > 0.95
     assert abs(prim.sum() - p_ref.sum()) <= max(3, 1e-3 * p_ref.sum())
+
+
+@pytest.mark.parametrize("label", [1, 2])
+@pytest.mark.parametrize("coherent", [0, 1])
+def test_mc_one_voxel_phantom(ctx, oracle, label, coherent):
+    """A phantom of one voxel: the flight box and the first-flight label sets reduce to that
+    voxel, and the detector sees it at a few pixels only.  Forward coherent scattering keeps
+    the photon's direction, so that mode also tallies scatter at those pixels."""
+    from monte_amd import geometry as G
+    from monte_amd.phantoms import MATERIALS, LabelPhantom
+
+    ph = LabelPhantom(np.full((1, 1, 1), label, np.uint8), (0.4, 0.4, 0.4), list(MATERIALS))
+    geom = G.small(n_views=3).replace(nu=16, nv=16, du=0.2, dv=0.2, nx=1, ny=1, nz=1, vx=0.4, vy=0.4, vz=0.4)
+    prim, scat, st, p_ref, s_ref, st_ref = _mc_pair(ctx, oracle, geom, ph, 20_000, e_kev=30.0, coherent=coherent)
+    print(label, coherent, "gpu", st, "cpu", st_ref)
+    assert st["histories"] == st_ref["histories"] == 20_000 * geom.n_views
+    assert st["photo"] + st["compton"] > 0
+    if coherent == 0:
+        assert st["tallied_scatter"] > 0
+    assert prim.sum() == st["tallied_primary"] and scat.sum() == st["tallied_scatter"]
+    assert np.mean(prim == p_ref) > 0.99 and np.mean(scat == s_ref) > 0.99
+    for k in ("tallied_primary", "tallied_scatter", "photo", "compton", "coherent"):
+        assert abs(st[k] - st_ref[k]) <= 3, k
 
 
 def test_mc_primary_matches_beer_lambert(ctx, oracle):
