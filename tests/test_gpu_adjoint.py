@@ -109,6 +109,30 @@ def test_projector_view_counts(ctx, oracle, n_views):
     assert abs(b1 - b2) <= 1e-5 * abs(b1)
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 33, 1), (40, 3, 17), (13, 29, 70)])
+def test_odd_volume_shapes(ctx, oracle, shape):
+    """Volumes (nx, ny, nz) of one voxel, one slice, a few rows, and taller than the detector's
+    field of view: partial backprojection tiles and z-chunks in every direction, and projector
+    column lists over slabs of one to 40 voxels, against the oracle's FP64 operators."""
+    from monte_amd.fdk import fdk_adjoint, fdk_backproject, fdk_project
+
+    g = _geoms()["std"].replace(nx=shape[0], ny=shape[1], nz=shape[2], n_views=12)
+    og = oracle_geom(oracle, g)
+    rng = np.random.default_rng(sum(shape))
+    f = rng.standard_normal(g.filt_shape).astype(np.float32)
+    v = fdk_backproject(_t(f), g, ctx).cpu().numpy()
+    v_ref = oracle.fdk_std_backproject(og, f)
+    y = rng.standard_normal(g.vol_shape).astype(np.float32)
+    a = fdk_adjoint(_t(y), g, ctx).cpu().numpy()
+    a_ref = oracle.fdk_std_adjoint(og, y)
+    print(shape, "backprojection", rel_max(v, v_ref), "adjoint", rel_max(a, a_ref), rel_l2(a, a_ref))
+    assert v.shape == v_ref.shape == g.vol_shape
+    assert rel_max(v, v_ref) < 1e-5
+    assert rel_max(a, a_ref) < 1e-5 and rel_l2(a, a_ref) < 1e-5
+    b1, b2 = _dot(v, y), _dot(f, fdk_project(_t(y), g, ctx).cpu().numpy())
+    assert abs(b1 - b2) <= 1e-5 * abs(b1)
+
+
 def test_autograd_matches_fp64_finite_differences(ctx, oracle):
     """d/dp of L = sum(g * FDK(p)) at 33^3 / 64^2 / 30 views."""
     import torch
