@@ -559,7 +559,15 @@ int prepare_mc(monte_ctx* c, std::vector<HostJob>& hj, int block, int blocks_per
   P.L = Lo;
   P.n = n;
   P.units = units;
+  // A chunk is one tile group over all of the first job's views.  A wave changes
+  // job only once its photons of the old job have all ended, so in a launch of
+  // several jobs a chunk holds at least 8 units per wave drawing from its queue:
+  // a wave then takes several units of one job in a row instead of a unit of
+  // another job almost every time.  64 uneven jobs (144k units): 13.7 ms with
+  // chunks of one job's views, 13.3, 13.0, 13.1, 12.7 and 13.2 ms with 1, 2, 4,
+  // 8 and 16 units per wave, the tallies bitwise the same.
   P.chunk = (uint32_t)std::max(1, hj[0].d.n_views_sim);
+  if (n > 1) P.chunk = std::max<uint32_t>(P.chunk, (uint32_t)(8 * grid_waves / kMcQueues));
   return MONTE_OK;
 }
 

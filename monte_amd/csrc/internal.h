@@ -181,7 +181,7 @@ struct McPrepared {
   McLayout L;
   int n = 0;
   uint64_t units = 0;
-  uint32_t chunk = 1;  // units per queue chunk (the first job's simulated views)
+  uint32_t chunk = 1;  // units per queue chunk (the first job's simulated views; more with several jobs)
   unsigned int* queue() const { return (unsigned int*)(base + L.q_off); }
   McCounters* counters() const { return (McCounters*)(base + L.cnt_off); }
 };
