@@ -140,7 +140,9 @@ typedef struct monte_mc_job {
 typedef struct monte_tuning {
   int32_t mc_block;          /* threads per block of the transport kernel */
   int32_t mc_batch;          /* photon cycles per work unit of one 8x8 detector tile (about 64 * mc_batch
-                                histories per unit; views with fewer cycles group tiles) */
+                                histories per unit; views with fewer cycles group tiles).  monte_mc_batch
+                                and monte_mc_project halve it, down to 16, for launches of fewer than 32
+                                units per wave of the grid */
   int32_t mc_blocks_per_cu;  /* persistent grid = CUs * this */
   int32_t bp_zchunk;         /* voxels along z per backprojection thread */
   int32_t bp_tile_x, bp_tile_y; /* backprojection block tile (threads) */
