@@ -27,7 +27,7 @@ def main(argv=None):
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--geom", choices=("full", "e2e"), default="full",
-                    help="fdk / adjoint geometry: full (325^3, 512^2) or e2e (512^3 at 0.4 mm, 1024 x 768, 720 views)")
+                    help="geometry (and MC phantom): full (325^3, 512^2) or e2e (512^3 at 0.4 mm, 1024 x 768, 720 views)")
     a = ap.parse_args(argv)
 
     import torch
@@ -50,7 +50,7 @@ def main(argv=None):
     if a.what == "mc":
         from .mc import McSettings, device_labels, mc_project
 
-        ph = phantoms.full_phantom()
+        ph = phantoms.full_phantom() if a.geom == "full" else phantoms.full_phantom(n=512, v=0.04)
         labels = device_labels(ph, dev)
         s = McSettings(photons_per_view=int(a.photons), spectrum=tables.kramers_spectrum(120.0, 2.5), coherent=1,
                        seed=20261202)
@@ -58,7 +58,7 @@ def main(argv=None):
             t = time.perf_counter()
             _, _, st = mc_project(ph, geom, s, labels_dev=labels, ctx=ctx)
             torch.cuda.synchronize()
-            print(f"mc pass {rep}: {a.views} views x {int(a.photons)} photons, kernel {st['kernel_ms']:.1f} ms, "
+            print(f"mc pass {rep}: {geom.n_views} views x {int(a.photons)} photons, kernel {st['kernel_ms']:.1f} ms, "
                   f"wall {(time.perf_counter() - t) * 1e3:.1f} ms", flush=True)
         return 0
 
@@ -75,7 +75,7 @@ def main(argv=None):
         t = time.perf_counter()
         fn(x, geom, ctx)
         torch.cuda.synchronize()
-        print(f"{a.what} pass {rep}: {a.views} views, wall {(time.perf_counter() - t) * 1e3:.1f} ms", flush=True)
+        print(f"{a.what} pass {rep}: {geom.n_views} views, wall {(time.perf_counter() - t) * 1e3:.1f} ms", flush=True)
     return 0
 
 
